@@ -706,11 +706,12 @@ static void t_gemm(fctx *c, int m, int n, int k)
     t->cap[iC] = outcap;
     t->rank[iC] = nr;
     /* operation count, as logged after the HCORE calls */
-    unsigned long qa = OPC('q', nn, s, 0, 0), qb;
-    if (n - k < band) qb = OPC('q', nn, s, 0, 0) + OPC('m', ra, nn, nn, 0);
-    else qb = OPC('m', ra, rb, nn, 0) + OPC('m', ra, nn, rb, 0) + OPC('q', nn, s, 0, 0);
+    /* counted with tile m's rows (tempmmv), as the reference does */
+    unsigned long qa = OPC('q', mm, s, 0, 0), qb;
+    if (n - k < band) qb = OPC('q', mm, s, 0, 0) + OPC('m', ra, mm, mm, 0);
+    else qb = OPC('m', ra, rb, mm, 0) + OPC('m', ra, mm, rb, 0) + OPC('q', mm, s, 0, 0);
     unsigned long sv = OPC('r', s, s, 2, 0) + OPC('s', s, 0, 0, 0) + (unsigned long)s * nr;
-    unsigned long nu = OPC('o', nn, nr, s, 1), nv = OPC('o', nr, nn, s, 2);
+    unsigned long nu = OPC('o', mm, nr, s, 1), nv = OPC('o', nr, mm, s, 2);
     logop(c, qa + qb + sv + nu + nv, m, n, 0, 0);
 }
 
