@@ -7,6 +7,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "wave.hpp"
 
 #ifndef BS
 #define BS 256
@@ -22,11 +23,35 @@
 // LDS scratch for reductions: NWAVE * QRB + QRB
 #define BLK_RED_SMEM (NWAVE * QRB + 2 * QRB)
 
-__device__ __forceinline__ double wave_sum(double v)
+__device__ __forceinline__ double wave_sum(double v) { return wsum(v); }
+
+// sum over aligned groups of g lanes, g a power of two <= 64 known at run time
+__device__ __forceinline__ double gsum_rt(double v, int g)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    switch (g) {
+    case 2: return gsum<2>(v);
+    case 4: return gsum<4>(v);
+    case 8: return gsum<8>(v);
+    case 16: return gsum<16>(v);
+    case 32: return gsum<32>(v);
+    case 64: return gsum<64>(v);
+    default: return v;
+    }
+}
+
+// Jacobi rotation (c, s) that zeroes the inner product of two columns with
+// squared norms al, be and inner product ga: t = tan(theta) with |t| <= 1.
+// Returns false when no rotation is needed or the values underflow.
+__device__ __forceinline__ bool jacobi_rot(double al, double be, double ga, double tol, double &c, double &s)
+{
+    if (ga == 0.0 || !(fabs(ga) > tol * sqrt(al * be))) return false;
+    const double diff = be - al;
+    const double den = fabs(diff) + sqrt(diff * diff + 4.0 * ga * ga);
+    if (!(den > 0.0)) return false;
+    const double t = copysign(2.0 * ga, diff) / den;
+    c = rsqrt(1.0 + t * t);
+    s = c * t;
+    return true;
 }
 
 // block-wide sum, same result (fixed order) on every thread
@@ -356,7 +381,7 @@ __device__ int blk_jacobi(int n, double *M, int ldm, double *Vs, int ldv, double
     int g = 64;
     while (g > 1 && g * npairs > BS) g >>= 1;
     const int gpb = BS / g;  // groups (pairs) handled concurrently
-    const double tol = 1.0e-15;
+    const double tol = 4.0e-16 * sqrt((double)n);
     int sweep = 0;
     for (; sweep < 60; sweep++) {
         if (tid == 0) *flag = 0;
@@ -380,17 +405,11 @@ __device__ int blk_jacobi(int n, double *M, int ldm, double *Vs, int ldv, double
                         al = fma(x, x, al); be = fma(y, y, be); ga = fma(x, y, ga);
                     }
                 }
-                for (int o = g >> 1; o > 0; o >>= 1) {
-                    al += __shfl_xor(al, o, 64);
-                    be += __shfl_xor(be, o, 64);
-                    ga += __shfl_xor(ga, o, 64);
-                }
-                if (a >= 0 && ga != 0.0 && fabs(ga) > tol * sqrt(al * be)) {
-                    double zeta = (be - al) / (2.0 * ga);
-                    double t;
-                    if (fabs(zeta) > 1.0e100) t = 0.5 / zeta;
-                    else t = copysign(1.0, zeta) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                al = gsum_rt(al, g);
+                be = gsum_rt(be, g);
+                ga = gsum_rt(ga, g);
+                double c, s;
+                if (a >= 0 && jacobi_rot(al, be, ga, tol, c, s)) {
                     double *ca = M + (size_t)a * ldm, *cb = M + (size_t)b * ldm;
                     for (int i = lane; i < n; i += g) {
                         double x = ca[i], y = cb[i];

@@ -4,6 +4,7 @@
 // replaced by a panel-step schedule of batched launches whose grids cover
 // every task of a step; tile ranks never leave HBM (kernels read them).
 #include "tlr_kernels.hpp"
+#include "rc_fast.hpp"
 #include "../../include/hicma_tlr.h"
 
 #include <math.h>
@@ -33,6 +34,8 @@ struct hicma_ctx {
     int *d_ovf;
     double *d_scalar;
     size_t lds_max;
+    int profile;                     // record events around recompression launches
+    std::vector<hipEvent_t> prof_ev; // pairs
 };
 
 struct hicma_tlr_desc {
@@ -43,7 +46,7 @@ struct hicma_tlr_desc {
     double *h_xyz;
     unsigned long long *d_tileops;
     unsigned long long *d_opstat;
-    unsigned long last_stats[4];
+    unsigned long last_stats[8];
     int last_ovf;
 };
 
@@ -143,7 +146,7 @@ extern "C" int hicma_init(hicma_ctx **ctx, int device)
     HIPCHK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return HICMA_ERR_ARG;
     HIPCHK(hipSetDevice(device));
-    hicma_ctx *c = (hicma_ctx *)calloc(1, sizeof(hicma_ctx));
+    hicma_ctx *c = new hicma_ctx();
     c->device = device;
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
@@ -164,8 +167,9 @@ extern "C" int hicma_fini(hicma_ctx *c)
     if (c->ws) hipFree(c->ws);
     hipFree(c->d_info);
     hipFree(c->d_scalar);
+    for (auto e : c->prof_ev) (void)hipEventDestroy(e);
     hipStreamDestroy(c->stream);
-    free(c);
+    delete c;
     return 0;
 }
 
@@ -212,7 +216,7 @@ extern "C" int hicma_desc_create(hicma_ctx *ctx, hicma_tlr_desc **out, int N, in
         hipMalloc(&t.rank, sizeof(int) * (size_t)NT * NT) != hipSuccess ||
         (lr_slots && hipMalloc(&t.lr, sizeof(double) * lr_slots * NB * 2 * maxrank) != hipSuccess) ||
         hipMalloc(&A->d_tileops, sizeof(unsigned long long) * (size_t)NT * NT) != hipSuccess ||
-        hipMalloc(&A->d_opstat, sizeof(unsigned long long) * 4) != hipSuccess) {
+        hipMalloc(&A->d_opstat, sizeof(unsigned long long) * 8) != hipSuccess) {
         hipGetLastError();
         if (dx) hipFree(dx);
         if (t.rank) hipFree(t.rank);
@@ -516,7 +520,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     const int NT = t.NT, NB = t.NB, cap = t.cap, b = band_size;
     HIPCHK(hipMemsetAsync(ctx->d_info, 0, 4 * sizeof(int), st));
     HIPCHK(hipMemsetAsync(A->d_tileops, 0, sizeof(unsigned long long) * (size_t)NT * NT, st));
-    HIPCHK(hipMemsetAsync(A->d_opstat, 0, sizeof(unsigned long long) * 4, st));
+    HIPCHK(hipMemsetAsync(A->d_opstat, 0, sizeof(unsigned long long) * 8, st));
     OpLog o{A->d_tileops, A->d_opstat};
 
     // workspaces: syrk (Y, Z per row), low-rank x dense gemm (W, X per pair),
@@ -535,7 +539,13 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     size_t syrk_d = (size_t)NT * NB * cap + (size_t)NT * cap * cap;
     size_t lrd_pairs = (b > 1) ? (size_t)NT * (b - 1) : 0;
     size_t lrd_d = lrd_pairs * ((size_t)cap * NB + (size_t)cap * cap);
-    size_t total = g.stride * ns + syrk_d + lrd_d;
+    // fast recompression path (1024 threads, one row per thread)
+    const bool fast = (NB <= FT);
+    size_t fstride = rcf_slot_doubles(NB);
+    int nsf = fast ? nslots_for(ctx, fstride * sizeof(double), maxtask > 0 ? maxtask : 1) : 0;
+    if (nsf > ctx->ncu) nsf = ctx->ncu;
+    size_t tasks_d = ((size_t)(maxtask > 0 ? maxtask : 1) + 1) / 2 + 1;  // ints packed in doubles
+    size_t total = g.stride * ns + syrk_d + lrd_d + fstride * nsf + tasks_d;
     int rc = ensure_ws(ctx, total * sizeof(double));
     if (rc) return rc;
     double *rcws = ctx->ws;
@@ -543,6 +553,11 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     double *Zws = Yws + (size_t)NT * NB * cap;
     double *Wws = Zws + (size_t)NT * cap * cap;
     double *Xws = Wws + lrd_pairs * cap * NB;
+    double *fws = Xws + lrd_pairs * cap * cap;
+    int *tasks_s = (int *)(fws + fstride * nsf);
+    if (fast)
+        HIPCHK(hipFuncSetAttribute((const void *)k_lr_gemm_fast, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(RCF_LDS * sizeof(double))));
     HIPCHK(hipFuncSetAttribute((const void *)k_lr_gemm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rc_lds));
 
     std::vector<hipEvent_t> ev;
@@ -585,8 +600,25 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
         long long ntask = (long long)T * (T - 1) / 2;
         if (ntask > 0 && T > b) {
             int grid = (int)std::min<long long>(ns, ntask);
-            hipLaunchKernelGGL(k_lr_gemm, dim3(grid), dim3(BS), rc_lds, st, t, k, rcws, g, acc, fixed_rank,
-                               maxrank, o, ctx->d_ovf);
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (ctx->profile) {
+                HIPCHK(hipEventCreate(&e0));
+                HIPCHK(hipEventCreate(&e1));
+                HIPCHK(hipEventRecord(e0, st));
+            }
+            hipLaunchKernelGGL(k_classify, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k, tasks_s);
+            if (fast) {
+                int gf = (int)std::min<long long>(nsf, ntask);
+                hipLaunchKernelGGL(k_lr_gemm_fast, dim3(gf), dim3(FT), RCF_LDS * sizeof(double), st, t, k,
+                                   (const int *)tasks_s, fws, fstride, acc, fixed_rank, maxrank, o, ctx->d_ovf);
+            }
+            hipLaunchKernelGGL(k_lr_gemm, dim3(grid), dim3(BS), rc_lds, st, t, k, (const int *)tasks_s,
+                               fast ? FS : -1, rcws, g, acc, fixed_rank, maxrank, o, ctx->d_ovf);
+            if (ctx->profile) {
+                HIPCHK(hipEventRecord(e1, st));
+                ctx->prof_ev.push_back(e0);
+                ctx->prof_ev.push_back(e1);
+            }
         }
         HIPCHK(hipGetLastError());
     }
@@ -594,9 +626,9 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     HIPCHK(hipStreamSynchronize(st));
     int hinfo[4];
     HIPCHK(hipMemcpy(hinfo, ctx->d_info, 4 * sizeof(int), hipMemcpyDeviceToHost));
-    unsigned long long stats[4];
+    unsigned long long stats[8];
     HIPCHK(hipMemcpy(stats, A->d_opstat, sizeof(stats), hipMemcpyDeviceToHost));
-    for (int i = 0; i < 4; i++) A->last_stats[i] = (unsigned long)stats[i];
+    for (int i = 0; i < 8; i++) A->last_stats[i] = (unsigned long)stats[i];
     A->last_ovf = hinfo[1];
     if (tileopcounters) {
         std::vector<unsigned long long> to((size_t)NT * NT);
@@ -625,6 +657,59 @@ extern "C" int hicma_op_stats(const hicma_tlr_desc *A, unsigned long *stats)
 {
     if (!A || !stats) return HICMA_ERR_ARG;
     for (int i = 0; i < 4; i++) stats[i] = A->last_stats[i];
+    return 0;
+}
+
+extern "C" int hicma_recompress_stats(const hicma_tlr_desc *A, unsigned long *stats)
+{
+    if (!A || !stats) return HICMA_ERR_ARG;
+    for (int i = 0; i < 3; i++) stats[i] = A->last_stats[4 + i];
+    return 0;
+}
+
+extern "C" int hicma_profile(hicma_ctx *ctx, int enable)
+{
+    if (!ctx) return HICMA_ERR_ARG;
+    for (auto e : ctx->prof_ev) (void)hipEventDestroy(e);
+    ctx->prof_ev.clear();
+    ctx->profile = enable;
+    return 0;
+}
+
+extern "C" int hicma_profile_read(hicma_ctx *ctx, double *ms_total, int *launches)
+{
+    if (!ctx) return HICMA_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double tot = 0.0;
+    for (size_t i = 0; i + 1 < ctx->prof_ev.size(); i += 2) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ctx->prof_ev[i], ctx->prof_ev[i + 1]));
+        tot += ms;
+    }
+    if (ms_total) *ms_total = tot;
+    if (launches) *launches = (int)(ctx->prof_ev.size() / 2);
+    return 0;
+}
+
+extern "C" int hicma_desc_copy(hicma_tlr_desc *dst, const hicma_tlr_desc *src)
+{
+    if (!dst || !src) return HICMA_ERR_ARG;
+    const TLR &a = src->t;
+    TLR &b = dst->t;
+    if (a.N != b.N || a.NB != b.NB || a.cap != b.cap || a.kind != b.kind) return HICMA_ERR_ARG;
+    hicma_ctx *c = dst->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(src->ctx->stream));
+    int rc = alloc_band(dst, a.band_size);
+    if (rc) return rc;
+    size_t nb2 = (size_t)a.NB * a.NB, slots = (size_t)a.NT * (a.NT - 1) / 2;
+    HIPCHK(hipMemcpyAsync(b.band, a.band, sizeof(double) * nb2 * a.NT * a.band_size, hipMemcpyDeviceToDevice,
+                          c->stream));
+    if (slots)
+        HIPCHK(hipMemcpyAsync(b.lr, a.lr, sizeof(double) * slots * a.NB * 2 * a.cap, hipMemcpyDeviceToDevice,
+                              c->stream));
+    HIPCHK(hipMemcpyAsync(b.rank, a.rank, sizeof(int) * (size_t)a.NT * a.NT, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
 

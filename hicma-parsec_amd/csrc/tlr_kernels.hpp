@@ -457,8 +457,8 @@ __host__ __device__ inline size_t rc_slot_doubles(int NB, int cap)
 // (iv) A low rank, B dense: Qn = -B Av
 // P = [Cu Au], Q = [Cv Qn]; QR both; M = Rp Rq^T; Jacobi SVD; keep
 // sigma >= acc (or the fixed rank) capped at maxrank; U' = Qp [Us S], V' = Qq [Vs].
-__global__ void __launch_bounds__(BS) k_lr_gemm(TLR t, int k, double *ws, RcWs g, double acc, int fixed_rk,
-                                                int maxrank, OpLog o, int *ovf)
+__global__ void __launch_bounds__(BS) k_lr_gemm(TLR t, int k, const int *tasks_s, int smin, double *ws, RcWs g,
+                                                double acc, int fixed_rk, int maxrank, OpLog o, int *ovf)
 {
     extern __shared__ double sm[];
     const int NB = t.NB, NT = t.NT, cap = t.cap, b = t.band_size, tid = threadIdx.x;
@@ -482,14 +482,15 @@ __global__ void __launch_bounds__(BS) k_lr_gemm(TLR t, int k, double *ws, RcWs g
     const int T = NT - 1 - k;
     const long long ntask = (long long)T * (T - 1) / 2;
     for (long long task = blockIdx.x; task < ntask; task += gridDim.x) {
+        // tasks_s: stack width decided before any update of this step; this
+        // kernel takes the tasks the fast path leaves (s > smin)
+        if (tasks_s[task] <= smin) continue;
         int hi, lo;
         tri_decode(task, hi, lo);
         const int m = k + 1 + hi, n = k + 1 + lo;
-        if (m - n < b) continue;
         const bool bdense = (n - k < b);
         const int ra = *rank_at(t, m, k), rc = *rank_at(t, m, n);
         const int rb = bdense ? 0 : *rank_at(t, n, k);
-        if (!bdense && (ra == 0 || rb == 0)) continue;
         const int s = rc + ra;
         const double *A = lr_tile(t, m, k), *Au = A, *Av = A + (size_t)NB * ra;
         double *C = lr_tile(t, m, n);
@@ -556,6 +557,12 @@ __global__ void __launch_bounds__(BS) k_lr_gemm(TLR t, int k, double *ws, RcWs g
             unsigned long long sv = opc('r', ss, ss, 2, 0) + opc('s', ss, 0, 0, 0) + ss * nr;
             unsigned long long nu = opc('o', nn, nr, ss, 1), nv = opc('o', nr, nn, ss, 2);
             oplog(o, NT, qa + qb + sv + nu + nv, m, n, false, false);
+            // recompression totals: op count, tasks, algorithmic bytes
+            // (8 * (inputs A, B, C + output) words, B dense in case iv)
+            atomicAdd(o.stat + 4, qa + qb + sv + nu + nv);
+            atomicAdd(o.stat + 5, 1ULL);
+            unsigned long long words = 2ULL * NB * (ra + rc + nr) + (bdense ? (unsigned long long)NB * NB : 2ULL * NB * rb);
+            atomicAdd(o.stat + 6, 8ULL * words);
         }
         __syncthreads();
     }

@@ -25,7 +25,8 @@ EXPORTS = [
     "hicma_get_points", "hicma_points", "STARSH_gen", "parsec_rank_gather", "parsec_rank_check",
     "HICMA_get_stat2", "parsec_band_size_auto_tuning", "hicma_band_autotune", "parsec_band_regenerate",
     "HiCMA_dpotrf_L", "hicma_op_counts", "hicma_op_stats", "hicma_rank_overflows", "STARSH_check",
-    "hicma_get_tile", "hicma_set_tile", "hicma_get_rank_map", "hicma_logdet",
+    "hicma_get_tile", "hicma_set_tile", "hicma_get_rank_map", "hicma_logdet", "hicma_recompress_stats",
+    "hicma_profile", "hicma_profile_read", "hicma_desc_copy",
 ]
 
 _lib = None
@@ -63,6 +64,10 @@ def lib():
             "hicma_set_tile": (i, [vp, i, i, dp, i]),
             "hicma_get_rank_map": (i, [vp, ip]),
             "hicma_logdet": (i, [vp, C.POINTER(d)]),
+            "hicma_recompress_stats": (i, [vp, vp]),
+            "hicma_profile": (i, [vp, i]),
+            "hicma_profile_read": (i, [vp, C.POINTER(d), C.POINTER(i)]),
+            "hicma_desc_copy": (i, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -109,6 +114,14 @@ class Context:
         if self.h:
             lib().hicma_fini(self.h)
             self.h = C.c_void_p()
+
+    def profile(self, enable=True):
+        _check(lib().hicma_profile(self.h, int(enable)), "hicma_profile")
+
+    def profile_read(self):
+        ms, n = C.c_double(), C.c_int()
+        _check(lib().hicma_profile_read(self.h, C.byref(ms), C.byref(n)), "hicma_profile_read")
+        return ms.value, n.value
 
     def __enter__(self):
         return self
@@ -193,6 +206,15 @@ class TLRMatrix:
         self.critical_path_time = cpt
         self.tileops = tile.reshape(self.NT, self.NT) if want_ops else None
         return info
+
+    def recompress_stats(self):
+        st = np.zeros(3, dtype=np.uint64)
+        _check(lib().hicma_recompress_stats(self.h, st.ctypes.data), "hicma_recompress_stats")
+        return int(st[0]), int(st[1]), int(st[2])
+
+    def copy_from(self, src):
+        _check(lib().hicma_desc_copy(self.h, src.h), "hicma_desc_copy")
+        self.band = src.band
 
     def rank_overflows(self):
         return lib().hicma_rank_overflows(self.h)

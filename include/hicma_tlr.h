@@ -103,6 +103,15 @@ unsigned long hicma_op_counts(char op, unsigned long a, unsigned long b, unsigne
 int hicma_op_stats(const hicma_tlr_desc *A, unsigned long *stats);
 /* number of recompressions whose rank was capped at maxrank in the last call */
 int hicma_rank_overflows(const hicma_tlr_desc *A);
+/* low-rank gemm + recompression totals of the last call: op count (op_counts
+ * formulas), number of tasks, algorithmic bytes (8 * words read + written) */
+int hicma_recompress_stats(const hicma_tlr_desc *A, unsigned long *stats);
+/* kernel timing: when enabled, every recompression launch is bracketed by
+ * HIP events on the context stream; read returns the summed milliseconds */
+int hicma_profile(hicma_ctx *ctx, int enable);
+int hicma_profile_read(hicma_ctx *ctx, double *ms_total, int *launches);
+/* copy tiles, band and rank map of src into dst (same N, NB, maxrank, kind) */
+int hicma_desc_copy(hicma_tlr_desc *dst, const hicma_tlr_desc *src);
 
 /* ---- STARSH_check (hicma_parsec/hicma_parsec.h:168-184): uncompress the
  * lower triangle into a host N x N column-major array (upper zeroed) ---- */

@@ -1,0 +1,82 @@
+"""CPU tests of the product library's host side: it loads, exports every entry
+point declared in include/hicma_tlr.h, and its host-only routines (point set,
+op counts, auto-tuning, rank statistics, argument errors) agree with the
+golden fixtures and the oracle.  No GPU call is made here."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import hicma_tlr as H
+import tlr_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+G = np.load(os.path.join(ROOT, "tests", "golden", "golden.npz"))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "hicma_tlr.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:unsigned long|int|void|double)\s+\**(\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = H.lib()
+    names = header_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(H.EXPORTS)
+
+
+@pytest.mark.parametrize("key,N,nd,seed", [("pts2_100", 100, 2, 0), ("pts3_64", 64, 3, 0),
+                                           ("pts2_2700", 2700, 2, 0), ("pts3_1000_s7", 1000, 3, 7)])
+def test_points_golden(key, N, nd, seed):
+    np.testing.assert_array_equal(H.points(N, nd, seed), G[key])
+
+
+def test_op_counts_match_oracle():
+    rng = np.random.default_rng(0)
+    for op in "qctmsor":
+        for _ in range(50):
+            a, b, c = (int(x) for x in rng.integers(1, 2000, size=3))
+            d = int(rng.integers(1, 3))
+            if op in "tr":
+                c = d
+            assert H.op_counts(op, a, b, c, d) == O.op_counts(op, a, b, c, d), (op, a, b, c, d)
+
+
+def test_band_autotune_matches_oracle():
+    rng = np.random.default_rng(5)
+    for NT, NB in [(10, 270), (30, 900), (16, 512)]:
+        for scale in (1, 5, 20, 60):
+            Gm = rng.integers(0, scale + 1, size=NT * NT).astype(np.int32)
+            assert H.band_autotune(Gm, NT, NB) == O.band_autotune(Gm, NT, NB)
+    assert H.band_autotune(np.zeros(100, np.int32), 10, 270) == 1
+    assert H.band_autotune(np.full(100, 135, np.int32), 10, 270) == 10
+
+
+def test_rank_stat_matches_oracle():
+    rng = np.random.default_rng(2)
+    Gm = rng.integers(0, 50, size=64).astype(np.int32)
+    for band in (1, 2, 3):
+        assert H.rank_stat(Gm, 8, band) == O.rank_stat(Gm, 8, band)
+
+
+def test_init_without_gpu_is_an_error_not_a_crash():
+    import ctypes as C
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is visible")
+    h = C.c_void_p()
+    assert H.lib().hicma_init(C.byref(h), 0) < 0
+
+
+def test_driver_usage():
+    exe = os.path.join(ROOT, "hicma-parsec_amd", "testing_dpotrf_tlr")
+    assert os.path.exists(exe)
+    r = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "--fixedacc" in r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
