@@ -472,18 +472,21 @@ extern "C" int parsec_band_regenerate(hicma_ctx *ctx, int uplo, hicma_tlr_desc *
 
 // ---------------------------------------------------------------------------
 // the factorization
-#define PB 64  // inner block of the diagonal-tile Cholesky
 
-static int launch_potrf(hicma_tlr_desc *A, int k)
+// blocked Cholesky of diagonal tile k: per 64-column block, factor + invert
+// the diagonal block in LDS, panel = C W^T, lower trailing update.  The block
+// inverses (Winv, NB/64 blocks of 64x64) are reused by the low-rank trsm.
+static int launch_potrf(hicma_tlr_desc *A, int k, double *Winv)
 {
     TLR &t = A->t;
     hipStream_t s = A->ctx->stream;
-    for (int j0 = 0; j0 < t.NB; j0 += PB) {
-        int jb = std::min(PB, t.NB - j0);
-        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(BS), 0, s, t, k, j0, jb, A->ctx->d_info);
+    for (int j0 = 0; j0 < t.NB; j0 += PBK) {
+        int jb = std::min(PBK, t.NB - j0);
+        hipLaunchKernelGGL(k_potrf_diag2, dim3(1), dim3(BS), 0, s, t, k, j0, jb, Winv, A->ctx->d_info);
         int R = t.NB - j0 - jb;
         if (R > 0) {
-            hipLaunchKernelGGL(k_potrf_panel, dim3((R + GT - 1) / GT), dim3(BS), 0, s, t, k, j0, jb);
+            hipLaunchKernelGGL(k_potrf_panel2, dim3((R + PBK - 1) / PBK), dim3(BS), 0, s, t, k, j0, jb,
+                               (const double *)Winv);
             int nt = (R + GT - 1) / GT;
             hipLaunchKernelGGL(k_potrf_update, dim3(nt * (nt + 1) / 2), dim3(BS), 0, s, t, k, j0, jb);
         }
@@ -545,7 +548,8 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     int nsf = fast ? nslots_for(ctx, fstride * sizeof(double), maxtask > 0 ? maxtask : 1) : 0;
     if (nsf > ctx->ncu) nsf = ctx->ncu;
     size_t tasks_d = ((size_t)(maxtask > 0 ? maxtask : 1) + 1) / 2 + 1;  // ints packed in doubles
-    size_t total = g.stride * ns + syrk_d + lrd_d + fstride * nsf + tasks_d;
+    size_t winv_d = (size_t)((NB + PBK - 1) / PBK) * PBK * PBK;
+    size_t total = g.stride * ns + syrk_d + lrd_d + fstride * nsf + tasks_d + winv_d;
     int rc = ensure_ws(ctx, total * sizeof(double));
     if (rc) return rc;
     double *rcws = ctx->ws;
@@ -554,7 +558,8 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     double *Wws = Zws + (size_t)NT * cap * cap;
     double *Xws = Wws + lrd_pairs * cap * NB;
     double *fws = Xws + lrd_pairs * cap * cap;
-    int *tasks_s = (int *)(fws + fstride * nsf);
+    double *Winv = fws + fstride * nsf;
+    int *tasks_s = (int *)(Winv + winv_d);
     if (fast)
         HIPCHK(hipFuncSetAttribute((const void *)k_lr_gemm_fast, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(RCF_LDS * sizeof(double))));
@@ -573,14 +578,14 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     const int ntl = ntg * (ntg + 1) / 2;
     for (int k = 0; k < NT; k++) {
         mark();
-        if ((rc = launch_potrf(A, k))) return rc;
+        if ((rc = launch_potrf(A, k, Winv))) return rc;
         mark();
         int nbnd = std::min(b - 1, NT - 1 - k);
         int nlr = NT - k - b;
         if (nbnd > 0)
             hipLaunchKernelGGL(k_trsm_band, dim3(ntg, nbnd), dim3(BS), 0, st, t, k, o);
         if (nlr > 0)
-            hipLaunchKernelGGL(k_trsm_lr, dim3((cap + TRC - 1) / TRC, nlr), dim3(BS), 0, st, t, k, o);
+            hipLaunchKernelGGL(k_trsm_lr2, dim3((cap + 15) / 16, nlr), dim3(BS), 0, st, t, k, (const double *)Winv, o);
         mark();
         if (nbnd > 0) hipLaunchKernelGGL(k_syrk_band, dim3(ntl, nbnd), dim3(BS), 0, st, t, k, o);
         if (nlr > 0) {

@@ -218,51 +218,6 @@ __global__ void __launch_bounds__(BS) k_compress(TLR t, const double *Om, double
 
 // ======================= dense potrf of a diagonal tile =======================
 
-// unblocked Cholesky of the jb x jb block at A(j0,j0) of tile (k,k)
-__global__ void __launch_bounds__(BS) k_potrf_diag(TLR t, int k, int j0, int jb, int *info)
-{
-    const int NB = t.NB, tid = threadIdx.x;
-    double *A = band_tile(t, k, k) + j0 + (size_t)j0 * NB;
-    __shared__ int bad;
-    if (tid == 0) bad = 0;
-    __syncthreads();
-    for (int j = 0; j < jb; j++) {
-        double d = A[j + (size_t)j * NB];
-        if (!(d > 0.0)) {
-            if (tid == 0) {
-                atomicCAS(info, 0, k * NB + j0 + j + 1);
-                bad = 1;
-            }
-            __syncthreads();
-            break;
-        }
-        d = sqrt(d);
-        __syncthreads();
-        if (tid == 0) A[j + (size_t)j * NB] = d;
-        for (int i = j + 1 + tid; i < jb; i += BS) A[i + (size_t)j * NB] /= d;
-        __syncthreads();
-        const int r = jb - j - 1;
-        for (int e = tid; e < r * r; e += BS) {
-            int l = j + 1 + e % r, c = j + 1 + e / r;
-            if (l >= c) A[l + (size_t)c * NB] -= A[l + (size_t)j * NB] * A[c + (size_t)j * NB];
-        }
-        __syncthreads();
-    }
-}
-
-// panel rows below the diagonal block: X L^T = C, grid = ceil(rows/GT)
-__global__ void __launch_bounds__(BS) k_potrf_panel(TLR t, int k, int j0, int jb)
-{
-    __shared__ double sm[BLK_GEMM_SMEM];
-    const int NB = t.NB;
-    double *A = band_tile(t, k, k);
-    int r0 = j0 + jb + blockIdx.x * GT;
-    int nr = NB - r0;
-    if (nr <= 0) return;
-    if (nr > GT) nr = GT;
-    blk_trsm_rltn(nr, jb, A + j0 + (size_t)j0 * NB, NB, A + r0 + (size_t)j0 * NB, NB, sm);
-}
-
 // trailing lower update S -= P P^T, grid = lower GT tiles of the trailing block
 __global__ void __launch_bounds__(BS) k_potrf_update(TLR t, int k, int j0, int jb)
 {
@@ -276,6 +231,162 @@ __global__ void __launch_bounds__(BS) k_potrf_update(TLR t, int k, int j0, int j
     const double *P = A + j1 + (size_t)j0 * NB;
     double *S = A + j1 + (size_t)j1 * NB;
     gemm_tile<false, true>(ti * GT, tj * GT, R, R, jb, -1.0, P, NB, P, NB, 1.0, S, NB, sm, true);
+}
+
+// ---- LDS-resident diagonal block: Cholesky + inverse ----
+// Factors the jb x jb block at A(j0,j0) of tile (k,k) in LDS, writes L back
+// and W = L^{-1} (ld PBK) to Winv[j0/PBK].  One workgroup of BS threads.
+#define PBK 64
+#define PLD (PBK + 1)
+__global__ void __launch_bounds__(BS) k_potrf_diag2(TLR t, int k, int j0, int jb, double *Winv, int *info)
+{
+    __shared__ double L[PBK * PLD];
+    __shared__ double W[PBK * PLD];
+    __shared__ int bad;
+    const int NB = t.NB, tid = threadIdx.x;
+    double *A = band_tile(t, k, k) + j0 + (size_t)j0 * NB;
+    for (int e = tid; e < jb * jb; e += BS) {
+        int i = e % jb, c = e / jb;
+        L[i + c * PLD] = (i >= c) ? A[i + (size_t)c * NB] : 0.0;
+    }
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < jb; j++) {
+        const double d2 = L[j + j * PLD];
+        if (!(d2 > 0.0)) {
+            if (tid == 0) { atomicCAS(info, 0, k * NB + j0 + j + 1); bad = 1; }
+            break;
+        }
+        const double d = sqrt(d2), rd = 1.0 / d;
+        __syncthreads();
+        if (tid == 0) L[j + j * PLD] = d;
+        for (int i = j + 1 + tid; i < jb; i += BS) L[i + j * PLD] *= rd;
+        __syncthreads();
+        const int r = jb - j - 1;
+        for (int e = tid; e < r * r; e += BS) {
+            int l = j + 1 + e % r, c = j + 1 + e / r;
+            if (l >= c) L[l + c * PLD] -= L[l + j * PLD] * L[c + j * PLD];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (bad) return;
+    // W = L^{-1}: 4 adjacent lanes per column, rows in sequence; lane q keeps
+    // the rows i with i % 4 == q of its column (same-lane LDS reuse)
+    {
+        const int c = tid >> 2, q = tid & 3;
+        const bool act = c < jb;
+        for (int i = 0; i < jb; i++) {
+            double s = 0.0;
+            if (act && i > c)
+                for (int l = c + ((q - c) & 3); l < i; l += 4) s = fma(L[i + l * PLD], W[l + c * PLD], s);
+            s = gsum<4>(s);
+            if (act && (i & 3) == q) {
+                double w = (i < c) ? 0.0 : (i == c ? 1.0 / L[c + c * PLD] : -s / L[i + i * PLD]);
+                W[i + c * PLD] = w;
+            }
+        }
+    }
+    __syncthreads();
+    double *Wo = Winv + (size_t)(j0 / PBK) * PBK * PBK;
+    for (int e = tid; e < jb * jb; e += BS) {
+        int i = e % jb, c = e / jb;
+        if (i >= c) A[i + (size_t)c * NB] = L[i + c * PLD];
+        Wo[i + c * PBK] = (i >= c) ? W[i + c * PLD] : 0.0;
+    }
+}
+
+// panel rows below the diagonal block: X = C W^T (W = L^{-1}), in place;
+// grid = ceil(rows / PBK)
+__global__ void __launch_bounds__(BS) k_potrf_panel2(TLR t, int k, int j0, int jb, const double *Winv)
+{
+    __shared__ double Cs[PBK * PLD];
+    __shared__ double Ws[PBK * PLD];
+    const int NB = t.NB, tid = threadIdx.x;
+    const int r0 = j0 + jb + blockIdx.x * PBK;
+    int nr = NB - r0;
+    if (nr <= 0) return;
+    if (nr > PBK) nr = PBK;
+    double *A = band_tile(t, k, k);
+    const double *Wi = Winv + (size_t)(j0 / PBK) * PBK * PBK;
+    for (int e = tid; e < PBK * jb; e += BS) {
+        int i = e % PBK, c = e / PBK;
+        Cs[i + c * PLD] = (i < nr) ? A[r0 + i + (size_t)(j0 + c) * NB] : 0.0;
+        if (i < jb) Ws[i + c * PLD] = Wi[i + c * PBK];
+    }
+    __syncthreads();
+    // thread: row i = tid % 64, columns c = tid / 64 + 4 q
+    const int i = tid % PBK, cg = tid / PBK;
+    double x[PBK / (BS / PBK)];
+#pragma unroll
+    for (int q = 0; q < PBK / (BS / PBK); q++) {
+        const int c = cg + q * (BS / PBK);
+        double s = 0.0;
+        if (c < jb)
+            for (int l = 0; l <= c; l++) s = fma(Cs[i + l * PLD], Ws[c + l * PLD], s);
+        x[q] = s;
+    }
+#pragma unroll
+    for (int q = 0; q < PBK / (BS / PBK); q++) {
+        const int c = cg + q * (BS / PBK);
+        if (c < jb && i < nr) A[r0 + i + (size_t)(j0 + c) * NB] = x[q];
+    }
+}
+
+// low-rank tiles m = k+band .. NT-1: V <- L^{-1} V by left-looking block
+// substitution with the inverted diagonal blocks of L; grid (ceil(cap/16), count)
+__global__ void __launch_bounds__(BS) k_trsm_lr2(TLR t, int k, const double *Winv, OpLog o)
+{
+    __shared__ double Vb[PBK * 17];
+    const int NB = t.NB, tid = threadIdx.x;
+    const int m = k + t.band_size + blockIdx.y;
+    const int r = *rank_at(t, m, k);
+    if (blockIdx.x == 0 && tid == 0) oplog(o, t.NT, opc('t', tdim(t, m), r, 1, 0), m, k, false, m - k == 1);
+    const int c0 = blockIdx.x * 16;
+    if (c0 >= r) return;
+    const int nc = (r - c0) < 16 ? (r - c0) : 16;
+    double *V = lr_tile(t, m, k) + (size_t)NB * r + (size_t)c0 * NB;
+    const double *L = band_tile(t, k, k);
+    const int i = tid % PBK, cg = tid / PBK;  // 4 column groups of 4
+    for (int b0 = 0; b0 < NB; b0 += PBK) {
+        const int nb = (NB - b0) < PBK ? (NB - b0) : PBK;
+        double acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int c = cg * 4 + q;
+            acc[q] = (i < nb && c < nc) ? V[b0 + i + (size_t)c * NB] : 0.0;
+        }
+        if (i < nb) {
+            const double *Lr = L + b0 + i;
+#pragma unroll 4
+            for (int l = 0; l < b0; l++) {
+                const double lv = Lr[(size_t)l * NB];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int c = cg * 4 + q;
+                    if (c < nc) acc[q] = fma(-lv, V[l + (size_t)c * NB], acc[q]);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) Vb[i + (cg * 4 + q) * PBK] = acc[q];
+        __syncthreads();
+        const double *Wb = Winv + (size_t)(b0 / PBK) * PBK * PBK;
+        double x[4] = {0.0, 0.0, 0.0, 0.0};
+        if (i < nb) {
+            for (int l = 0; l <= i; l++) {
+                const double w = Wb[i + (size_t)l * PBK];
+#pragma unroll
+                for (int q = 0; q < 4; q++) x[q] = fma(w, Vb[l + (cg * 4 + q) * PBK], x[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int c = cg * 4 + q;
+            if (i < nb && c < nc) V[b0 + i + (size_t)c * NB] = x[q];
+        }
+        __syncthreads();
+    }
 }
 
 // ======================= trsm =======================
@@ -293,23 +404,6 @@ __global__ void __launch_bounds__(BS) k_trsm_band(TLR t, int k, OpLog o)
         oplog(o, t.NT, opc('t', mm, mm, 1, 0), m, k, true, m - k == 1);
     }
     blk_trsm_rltn(nr, NB, band_tile(t, k, k), NB, band_tile(t, m, k) + r0, NB, sm);
-}
-
-#define TRC 16
-// low-rank tiles m = k+band .. NT-1: V <- L^{-1} V; grid (ceil(cap/TRC), count)
-__global__ void __launch_bounds__(BS) k_trsm_lr(TLR t, int k, OpLog o)
-{
-    __shared__ double sm[BLK_GEMM_SMEM];
-    const int NB = t.NB;
-    int m = k + t.band_size + blockIdx.y;
-    int r = *rank_at(t, m, k);
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        oplog(o, t.NT, opc('t', tdim(t, m), r, 1, 0), m, k, false, m - k == 1);
-    int c0 = blockIdx.x * TRC;
-    if (c0 >= r) return;
-    int nc = r - c0 < TRC ? r - c0 : TRC;
-    double *V = lr_tile(t, m, k) + (size_t)NB * r + (size_t)c0 * NB;
-    blk_trsm_llnn(NB, nc, band_tile(t, k, k), NB, V, NB, sm);
 }
 
 // ======================= syrk =======================

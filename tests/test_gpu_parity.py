@@ -1,0 +1,135 @@
+"""GPU parity beyond the reference's CTest size: ragged tiles, the 3-D kernels,
+fixed-rank recompression, forced 2-flow, error paths, determinism and the
+reference op counts.  The oracle (oracle/) is the checker throughout."""
+import numpy as np
+import pytest
+
+import hicma_tlr as H
+import tlr_oracle as O
+
+pytestmark = pytest.mark.gpu
+EPS = 1e-8
+
+
+def factor_and_check(ctx, N, NB, U, kind, beta, nugget=1e-2, auto=True, fixed_rank=0, two_flow=0, sigma2=1.0):
+    g = H.TLRMatrix(ctx, N, NB, U, kind, sigma2, beta, nugget, 0)
+    assert g.generate(EPS, U, 1) == 0
+    band = 1
+    if auto:
+        band = g.auto_tune()
+        if band != 1:
+            g.regenerate(band)
+    info = g.potrf(EPS, fixed_rank, U, two_flow=two_flow, want_ops=True)
+    o = O.Oracle(N, NB, kind, sigma2, beta, nugget, 0)
+    A = o.dense_exact()
+    L = g.uncompress()
+    return g, info, band, L, A
+
+
+@pytest.mark.parametrize("N,NB,U", [(2600, 270, 130), (1999, 200, 100)])
+def test_ragged_last_tile(ctx, N, NB, U):
+    g, info, band, L, A = factor_and_check(ctx, N, NB, U, H.SQEXP_2D, 0.1)
+    assert info == 0
+    assert O.check_dpotrf2(L, A) <= 10 * EPS
+    Ld, _ = O.dense_potrf(A)
+    assert abs(g.logdet() - O.dense_logdet(Ld)) <= 1e-8 * abs(O.dense_logdet(Ld))
+
+
+@pytest.mark.parametrize("kind,beta", [(H.EXP_3D, 0.1), (H.SQEXP_3D, 0.05)])
+def test_3d_kernels(ctx, kind, beta):
+    N, NB, U = 4096, 512, 256
+    g, info, band, L, A = factor_and_check(ctx, N, NB, U, kind, beta)
+    assert info == 0
+    res = O.check_dpotrf2(L, A)
+    print(f"kind={kind} band={band} residual={res:.3e}")
+    assert res <= 10 * EPS
+
+
+def test_fixed_rank_mode(ctx):
+    # -f: every recompression keeps exactly min(rk, s) columns
+    N, NB, U, rk = 2700, 270, 130, 40
+    g, info, band, L, A = factor_and_check(ctx, N, NB, U, H.SQEXP_2D, 0.1, auto=False, fixed_rank=rk)
+    assert info == 0
+    R = g.rank_map()
+    off = [R[m, n] for m in range(g.NT) for n in range(m) if m - n >= 1 and n >= 1]
+    assert max(off) <= rk
+    assert O.check_dpotrf2(L, A) <= 1e-5
+
+
+def test_forced_two_flow_equals_three_flow(ctx):
+    N, NB, U = 2700, 270, 130
+    a = H.TLRMatrix(ctx, N, NB, U, H.SQEXP_2D, 1.0, 0.1, 1e-2, 0)
+    b = H.TLRMatrix(ctx, N, NB, U, H.SQEXP_2D, 1.0, 0.1, 1e-2, 0)
+    assert a.generate(EPS, U, 1) == 0 and b.generate(EPS, U, 1) == 0
+    assert a.potrf(EPS, 0, U, two_flow=0) == 0 and a.two_flow == 0
+    assert b.potrf(EPS, 0, U, two_flow=1) == 0 and b.two_flow == 1
+    np.testing.assert_array_equal(a.rank_map(), b.rank_map())
+    np.testing.assert_array_equal(a.uncompress(), b.uncompress())
+
+
+def test_deterministic(ctx):
+    N, NB, U = 2700, 270, 130
+    outs = []
+    for _ in range(2):
+        g = H.TLRMatrix(ctx, N, NB, U, H.SQEXP_2D, 1.0, 0.1, 1e-2, 0)
+        assert g.generate(EPS, U, 1) == 0
+        assert g.potrf(EPS, 0, U) == 0
+        outs.append((g.rank_map().copy(), g.uncompress(), g.logdet()))
+        g.close()
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    assert outs[0][2] == outs[1][2]
+
+
+def test_op_counts_follow_reference_formulas(ctx):
+    """Op counts accumulated on the device equal the oracle's for the same
+    rank trajectory: run the oracle on the GPU-generated input and compare
+    totals (ranks may differ by rounding, so allow 2%)."""
+    N, NB, U = 2700, 270, 130
+    g = H.TLRMatrix(ctx, N, NB, U, H.SQEXP_2D, 1.0, 0.1, 1e-2, 0)
+    assert g.generate(EPS, U, 1) == 0
+    o = O.Oracle(N, NB, 2, 1.0, 0.1, 1e-2, 0)
+    o.set_band(1)
+    for n in range(g.NT):
+        for m in range(n, g.NT):
+            d, r = g.get_tile(m, n)
+            if m == n:
+                o.set_tile(m, n, np.asfortranarray(d).ravel(order="F"), 0)
+            else:
+                o.set_tile(m, n, np.concatenate([d[0].ravel(order="F"), d[1].ravel(order="F")]), r)
+    assert g.potrf(EPS, 0, U, want_ops=True) == 0
+    assert o.potrf(EPS, 0, U) == 0
+    st = g.stats
+    assert st[0] + st[1] == st[2] + st[3] == g.tileops.sum()
+    # potrf / syrk / trsm counts depend on ranks only through the trsm/syrk terms
+    assert abs(int(st[0] + st[1]) - int(o.stats[0] + o.stats[1])) <= 0.02 * int(o.stats[0] + o.stats[1])
+    assert g.tileops[0, 0] == o.tileops[0, 0] == O.op_counts("c", NB)
+
+
+def test_nonspd_reports_row(ctx):
+    g = H.TLRMatrix(ctx, 2700, 270, 130, H.SQEXP_2D, 1.0, 0.1, -2.0, 0)
+    assert g.generate(EPS, 130, 1) == 0
+    assert g.potrf(EPS, 0, 130) == 1
+
+
+def test_generation_rank_overflow_is_info_minus_one(ctx):
+    g = H.TLRMatrix(ctx, 2700, 270, 130, H.SQEXP_2D, 1.0, 0.1, 1e-2, 0)
+    assert g.generate(EPS, 5, 1) == -1
+
+
+def test_argument_errors(ctx):
+    g = H.TLRMatrix(ctx, 2700, 270, 130, H.SQEXP_2D, 1.0, 0.1, 1e-2, 0)
+    assert g.generate(EPS, 130, 1) == 0
+    import ctypes as C
+    la, tf = C.c_int(-1), C.c_int(0)
+    L = H.lib()
+    # maxrank > NB/2
+    assert L.HiCMA_dpotrf_L(ctx.h, H.LOWER, g.h, g.h, g.h, EPS, 0, 136, C.byref(la), 1, 300, 136, 0,
+                            C.byref(tf), None, None, None, 0) == -2
+    # maxrank > compmaxrank
+    assert L.HiCMA_dpotrf_L(ctx.h, H.LOWER, g.h, g.h, g.h, EPS, 0, 100, C.byref(la), 1, 300, 50, 0,
+                            C.byref(tf), None, None, None, 0) == -3
+    # uplo Upper
+    assert L.HiCMA_dpotrf_L(ctx.h, H.UPPER, g.h, g.h, g.h, EPS, 0, 100, C.byref(la), 1, 300, 100, 0,
+                            C.byref(tf), None, None, None, 0) == -1
+    assert la.value == -1 or la.value == 1
