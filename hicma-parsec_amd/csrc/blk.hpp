@@ -48,7 +48,8 @@ __device__ __forceinline__ bool jacobi_rot(double al, double be, double ga, doub
     const double diff = be - al;
     const double den = fabs(diff) + sqrt(diff * diff + 4.0 * ga * ga);
     if (!(den > 0.0)) return false;
-    const double t = copysign(2.0 * ga, diff) / den;
+    // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), zeta = diff / (2 ga)
+    const double t = (diff >= 0.0 ? 2.0 * ga : -2.0 * ga) / den;
     c = rsqrt(1.0 + t * t);
     s = c * t;
     return true;

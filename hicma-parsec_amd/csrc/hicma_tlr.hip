@@ -5,6 +5,7 @@
 // every task of a step; tile ranks never leave HBM (kernels read them).
 #include "tlr_kernels.hpp"
 #include "rc_fast.hpp"
+#include "rc_wave.hpp"
 #include "../../include/hicma_tlr.h"
 
 #include <math.h>
@@ -476,13 +477,13 @@ extern "C" int parsec_band_regenerate(hicma_ctx *ctx, int uplo, hicma_tlr_desc *
 // blocked Cholesky of diagonal tile k: per 64-column block, factor + invert
 // the diagonal block in LDS, panel = C W^T, lower trailing update.  The block
 // inverses (Winv, NB/64 blocks of 64x64) are reused by the low-rank trsm.
-static int launch_potrf(hicma_tlr_desc *A, int k, double *Winv)
+static int launch_potrf(hicma_tlr_desc *A, int k, double *Winv, OpLog o)
 {
     TLR &t = A->t;
     hipStream_t s = A->ctx->stream;
     for (int j0 = 0; j0 < t.NB; j0 += PBK) {
         int jb = std::min(PBK, t.NB - j0);
-        hipLaunchKernelGGL(k_potrf_diag2, dim3(1), dim3(BS), 0, s, t, k, j0, jb, Winv, A->ctx->d_info);
+        hipLaunchKernelGGL(k_potrf_diag2, dim3(1), dim3(BS), 0, s, t, k, j0, jb, Winv, A->ctx->d_info, o);
         int R = t.NB - j0 - jb;
         if (R > 0) {
             hipLaunchKernelGGL(k_potrf_panel2, dim3((R + PBK - 1) / PBK), dim3(BS), 0, s, t, k, j0, jb,
@@ -549,7 +550,15 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     if (nsf > ctx->ncu) nsf = ctx->ncu;
     size_t tasks_d = ((size_t)(maxtask > 0 ? maxtask : 1) + 1) / 2 + 1;  // ints packed in doubles
     size_t winv_d = (size_t)((NB + PBK - 1) / PBK) * PBK * PBK;
-    size_t total = g.stride * ns + syrk_d + lrd_d + fstride * nsf + tasks_d + winv_d;
+    // one-wave recompression (tasks with s <= WV_SMAX): 2 workgroups per CU of
+    // the s <= 48 variant (4 waves each) and of the s <= 64 variant (2 waves)
+    const bool wave = (NB <= 64 * WV_KMAX);
+    const int wgrid = wave ? 2 * ctx->ncu : 0;
+    const size_t wstride = wv_slot_doubles(NB);
+    const int wcap = (int)(maxtask > 0 ? maxtask : 1);
+    size_t wws_d = (size_t)wgrid * WvCfg<48>::WPB * wstride;
+    size_t wlist_d = ((size_t)2 * wcap + 4) / 2 + 1;
+    size_t total = g.stride * ns + syrk_d + lrd_d + fstride * nsf + tasks_d + winv_d + wws_d + wlist_d;
     int rc = ensure_ws(ctx, total * sizeof(double));
     if (rc) return rc;
     double *rcws = ctx->ws;
@@ -560,10 +569,21 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     double *fws = Xws + lrd_pairs * cap * cap;
     double *Winv = fws + fstride * nsf;
     int *tasks_s = (int *)(Winv + winv_d);
+    double *wws = Winv + winv_d + tasks_d;
+    int *wcount = (int *)(wws + wws_d);
+    int *wlist = wcount + 4;
     if (fast)
         HIPCHK(hipFuncSetAttribute((const void *)k_lr_gemm_fast, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(RCF_LDS * sizeof(double))));
     HIPCHK(hipFuncSetAttribute((const void *)k_lr_gemm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rc_lds));
+    const size_t wlds48 = sizeof(double) * WvCfg<48>::LDS * WvCfg<48>::WPB;
+    const size_t wlds64 = sizeof(double) * WvCfg<64>::LDS * WvCfg<64>::WPB;
+    if (wave) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_rc_wave<48>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)wlds48));
+        HIPCHK(hipFuncSetAttribute((const void *)k_rc_wave<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)wlds64));
+    }
 
     std::vector<hipEvent_t> ev;
     bool timing = (critical_path_time != nullptr);
@@ -578,7 +598,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     const int ntl = ntg * (ntg + 1) / 2;
     for (int k = 0; k < NT; k++) {
         mark();
-        if ((rc = launch_potrf(A, k, Winv))) return rc;
+        if ((rc = launch_potrf(A, k, Winv, o))) return rc;
         mark();
         int nbnd = std::min(b - 1, NT - 1 - k);
         int nlr = NT - k - b;
@@ -612,6 +632,19 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
                 HIPCHK(hipEventRecord(e0, st));
             }
             hipLaunchKernelGGL(k_classify, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k, tasks_s);
+            if (wave) {
+                HIPCHK(hipMemsetAsync(wcount, 0, 2 * sizeof(int), st));
+                hipLaunchKernelGGL(k_classify_wave, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k,
+                                   tasks_s, wlist, wcount, wcap);
+                int wg = (int)std::min<long long>(wgrid, (ntask + WvCfg<48>::WPB - 1) / WvCfg<48>::WPB);
+                hipLaunchKernelGGL(k_rc_wave<48>, dim3(wg), dim3(64 * WvCfg<48>::WPB), wlds48, st, t, k,
+                                   (const int *)wlist, (const int *)wcount, wws, acc, fixed_rank, maxrank, o,
+                                   ctx->d_ovf);
+                wg = (int)std::min<long long>(wgrid, (ntask + WvCfg<64>::WPB - 1) / WvCfg<64>::WPB);
+                hipLaunchKernelGGL(k_rc_wave<64>, dim3(wg), dim3(64 * WvCfg<64>::WPB), wlds64, st, t, k,
+                                   (const int *)(wlist + wcap), (const int *)(wcount + 1), wws, acc, fixed_rank,
+                                   maxrank, o, ctx->d_ovf);
+            }
             if (fast) {
                 int gf = (int)std::min<long long>(nsf, ntask);
                 hipLaunchKernelGGL(k_lr_gemm_fast, dim3(gf), dim3(FT), RCF_LDS * sizeof(double), st, t, k,

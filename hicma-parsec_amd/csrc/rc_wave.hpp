@@ -1,0 +1,597 @@
+// rc_wave.hpp -- recompression of a low-rank gemm update by ONE wave.
+//
+// A task is the update C_mn -= A_mk A_nk^T of three low-rank tiles (the
+// reference's hcore_dgemm QR + SVD recompression, hicma_parsec_core.c).  The
+// wave owns its task from start to end: every reduction over the NB tile rows
+// is a wave reduction (permlane / DPP / readlane), LDS is private to the wave
+// and no barrier is shared with other waves, so a CU works on eight tasks at
+// once and a step with thousands of tasks fills all CUs.
+//
+// Mathematics (the reference's up to rounding):
+//   X = Av^T Bv,  Qn = -Bu X^T,  P = [Cu Au],  Q = [Cv Qn]        (NB x s)
+//   Rp, Rq = R factors of P, Q (Householder, panels of WV_PW columns)
+//   M = Rp Rq^T (s x s);  one-sided Jacobi on the rows of M:  M^T J = Vs S
+//   -> right singular vectors Vs and singular values S of M;
+//   r' = #{S >= acc} (or the fixed rank), capped at maxrank;
+//   U' = P (Rq^T Vs_r)   ( = Qp Us_r S_r because P = Qp Rp, M Vs = Us S )
+//   V' = Q (Rq^-1 Vs_r)  ( = Qq Vs_r     because Q = Qq Rq )
+// so neither orthogonal factor is ever formed.
+//
+// Layout: lane l holds rows l, l+64, ... of every tall (NB-row) operand, so
+// tall data is read and written coalesced and row-locally.
+#pragma once
+#include "tlr_kernels.hpp"
+
+#define WV_SMAX 64   // largest stack width s = rc + ra, and largest rb
+#define WV_KMAX 16   // rows per lane: NB <= 64 * WV_KMAX
+#define WV_PW 4      // Householder panel width
+
+// Per-variant sizes: SM = largest stack width of the variant (48 or 64).
+template <int SM>
+struct WvCfg {
+    static constexpr int LDJ = SM + 2;             // Jacobi column stride in LDS
+    static constexpr int SIG = SM * LDJ;           // offset of the singular values
+    static constexpr int LDS = SIG + 2 * SM;       // doubles of LDS per wave
+    static constexpr int WPB = SM <= 48 ? 4 : 2;   // waves (tasks in flight) per workgroup
+    static constexpr int MINW = SM <= 48 ? 2 : 1;  // waves per SIMD the registers must allow
+};
+
+// per-wave HBM workspace: W (QR copy), Qn, Out (2 blocks), Rp, Rq, Ku
+__host__ __device__ inline size_t wv_slot_doubles(int NB)
+{
+    return (size_t)NB * 4 * WV_SMAX + 4 * (size_t)WV_SMAX * WV_SMAX;
+}
+
+// LDS ordering inside one wave: a code-motion barrier plus wavefront-scope
+// fences (LDS operations of one wave complete in order)
+__device__ __forceinline__ void wv_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double rdlane(double v, int l)
+{
+    return mk64((unsigned)__builtin_amdgcn_readlane((int)lo32(v), l),
+                (unsigned)__builtin_amdgcn_readlane((int)hi32(v), l));
+}
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// reduce-scatter of 4 values: lane l returns the wave total of value l >> 4
+__device__ __forceinline__ double wrs4(double p0, double p1, double p2, double p3)
+{
+    double x0 = p0, y0 = p2, x1 = p1, y1 = p3;
+    swap32d(x0, y0);  // lanes < 32: values 0, 1; lanes >= 32: values 2, 3
+    swap32d(x1, y1);
+    double a = x0 + y0, b = x1 + y1;
+    swap16d(a, b);    // value index = lane >> 4
+    return gsum<16>(a + b);
+}
+
+// Column c of the stack: first pass from the tiles, afterwards from W.
+struct WvCols {
+    const double *src0, *src1;
+    int c0;
+    double *W;
+    int NB;
+    __device__ __forceinline__ const double *col(int c, bool first) const
+    {
+        if (!first) return W + (size_t)c * NB;
+        return c < c0 ? src0 + (size_t)c * NB : src1 + (size_t)(c - c0) * NB;
+    }
+};
+
+// R factor of the NB x s stack (s <= SM <= 64, so pivot row j is on lane
+// j of the first row block).  Panels of WV_PW columns live in registers,
+// trailing columns stream through W.  R (row-major, ld WV_SMAX, zeros below
+// the diagonal) is written to Rout.
+template <int SM>
+__device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
+{
+    const int lane = threadIdx.x & 63;
+    const int NB = A.NB;
+    for (int j0 = 0; j0 < s; j0 += WV_PW) {
+        const int jb = min(WV_PW, s - j0);
+        const bool first = (j0 == 0);
+        double a[WV_KMAX][WV_PW];
+#pragma unroll
+        for (int c = 0; c < WV_PW; c++) {
+            const double *col = A.col(j0 + c, first);
+#pragma unroll
+            for (int k = 0; k < WV_KMAX; k++) {
+                const int i = lane + 64 * k;
+                a[k][c] = (c < jb && i < NB) ? col[i] : 0.0;
+            }
+        }
+        double T[WV_PW][WV_PW];
+#pragma unroll
+        for (int r = 0; r < WV_PW; r++)
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++) T[r][c] = 0.0;
+        // ---- panel factorization ----
+#pragma unroll
+        for (int jj = 0; jj < WV_PW; jj++) {
+            if (jj < jb) {
+                const int j = j0 + jj;
+                // x = column jj; y = its part strictly below the pivot row j
+                const double x0 = a[0][jj];
+                const double y0 = (lane > j) ? x0 : 0.0;
+                double p[WV_PW];
+#pragma unroll
+                for (int c = 0; c < WV_PW; c++) p[c] = y0 * (c == jj ? y0 : a[0][c]);
+#pragma unroll
+                for (int k = 1; k < WV_KMAX; k++)
+#pragma unroll
+                    for (int c = 0; c < WV_PW; c++) p[c] = fma(a[k][jj], a[k][c], p[c]);
+                const double red = wrs4(p[0], p[1], p[2], p[3]);
+                double S[WV_PW], prow[WV_PW];
+#pragma unroll
+                for (int c = 0; c < WV_PW; c++) {
+                    S[c] = rdlane(red, 16 * c);
+                    prow[c] = rdlane(a[0][c], j);
+                }
+                const double alpha = prow[jj], xn2 = S[jj];
+                double tau = 0.0, beta = alpha, scal = 0.0;
+                if (xn2 > 0.0) {
+                    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+                    tau = (beta - alpha) / beta;
+                    scal = 1.0 / (alpha - beta);
+                }
+                // reflector v in place of column jj: 1 at the pivot, y * scal below
+                a[0][jj] = (lane == j) ? 1.0 : y0 * scal;
+#pragma unroll
+                for (int k = 1; k < WV_KMAX; k++) a[k][jj] *= scal;
+#pragma unroll
+                for (int c = 0; c < WV_PW; c++) {
+                    if (c > jj) {
+                        const double td = tau * (prow[c] + S[c] * scal);  // tau * v^T a_c
+#pragma unroll
+                        for (int k = 0; k < WV_KMAX; k++) a[k][c] = fma(-td, a[k][jj], a[k][c]);
+                    }
+                }
+                // row block 0 of column jj: beta at the pivot, R entries above it
+                a[0][jj] = (lane == j) ? beta : ((lane > j) ? a[0][jj] : x0);
+                // T(0:jj, jj) = -tau T(0:jj, 0:jj) w,  w_q = v_q^T v_jj = prow[q] + S[q] * scal
+#pragma unroll
+                for (int l = 0; l < WV_PW; l++) {
+                    if (l < jj) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int q = 0; q < WV_PW; q++)
+                            if (q >= l && q < jj) acc = fma(T[l][q], prow[q] + S[q] * scal, acc);
+                        T[l][jj] = -tau * acc;
+                    }
+                }
+                T[jj][jj] = tau;
+            }
+        }
+        // R entries of the panel columns: rows <= j0 + c on lanes of row block 0
+#pragma unroll
+        for (int c = 0; c < WV_PW; c++)
+            if (c < jb && lane < s) Rout[lane * SM + j0 + c] = (lane <= j0 + c) ? a[0][c] : 0.0;
+        // reflector entries of row block 0: 1 at the pivot, 0 above
+        double v0[WV_PW];
+#pragma unroll
+        for (int c = 0; c < WV_PW; c++)
+            v0[c] = (c < jb) ? ((lane == j0 + c) ? 1.0 : (lane > j0 + c ? a[0][c] : 0.0)) : 0.0;
+        // ---- trailing columns: b -= V (T^T (V^T b)) ----
+        for (int t = j0 + jb; t < s; t++) {
+            const double *col = A.col(t, first);
+            double b[WV_KMAX];
+#pragma unroll
+            for (int k = 0; k < WV_KMAX; k++) {
+                const int i = lane + 64 * k;
+                b[k] = (i < NB) ? col[i] : 0.0;
+            }
+            double p[WV_PW];
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++) {
+                double acc = v0[c] * b[0];
+#pragma unroll
+                for (int k = 1; k < WV_KMAX; k++) acc = fma(a[k][c], b[k], acc);
+                p[c] = acc;
+            }
+            const double red = wrs4(p[0], p[1], p[2], p[3]);
+            double d[WV_PW], w[WV_PW];
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++) d[c] = rdlane(red, 16 * c);
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int q = 0; q <= c; q++) acc = fma(T[q][c], d[q], acc);
+                w[c] = acc;
+            }
+            {
+                double acc = b[0];
+#pragma unroll
+                for (int c = 0; c < WV_PW; c++) acc = fma(-v0[c], w[c], acc);
+                b[0] = acc;
+            }
+#pragma unroll
+            for (int k = 1; k < WV_KMAX; k++) {
+                double acc = b[k];
+#pragma unroll
+                for (int c = 0; c < WV_PW; c++) acc = fma(-a[k][c], w[c], acc);
+                b[k] = acc;
+            }
+            double *out = A.W + (size_t)t * NB;
+#pragma unroll
+            for (int k = 0; k < WV_KMAX; k++) {
+                const int i = lane + 64 * k;
+                if (i < NB) out[i] = b[k];
+            }
+        }
+    }
+}
+
+// Tasks of step k whose update runs on one wave: both factors low rank,
+// s = rc + ra in [1, WV_SMAX], rb <= WV_SMAX, NB <= 64 * WV_KMAX.  They are
+// appended to list 0 (s, rb <= 48) or list 1 (up to WV_SMAX); the order inside
+// a list is irrelevant (tasks of one step are independent).  Listed tasks are
+// marked -1 in tasks_s; every other task keeps its k_classify value.
+__global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wcount, int cap)
+{
+    const int T = t.NT - 1 - k;
+    const long long ntask = (long long)T * (T - 1) / 2;
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ntask) return;
+    int hi, lo;
+    tri_decode(e, hi, lo);
+    const int m = k + 1 + hi, n = k + 1 + lo, b = t.band_size;
+    if (m - n < b || n - k < b || t.NB > 64 * WV_KMAX) return;
+    const int ra = *rank_at(t, m, k), rc = *rank_at(t, m, n), rb = *rank_at(t, n, k);
+    const int s = rc + ra;
+    if (ra > 0 && rb > 0 && s <= WV_SMAX && rb <= WV_SMAX) {
+        tasks_s[e] = -1;
+        const int l = (s <= 48 && rb <= 48) ? 0 : 1;
+        wlist[(size_t)l * cap + atomicAdd(wcount + l, 1)] = (int)e;
+    }
+}
+
+template <int SM>
+__global__ void __launch_bounds__(64 * WvCfg<SM>::WPB, WvCfg<SM>::MINW)
+k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, double *ws, double acc, int fixed_rk, int maxrank,
+          OpLog o, int *ovf)
+{
+    using Cf = WvCfg<SM>;
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x & 63;
+    const int wv = uni(threadIdx.x >> 6);
+    double *L = smem + (size_t)wv * Cf::LDS;
+    double *sg = L + Cf::SIG;
+    const int NB = t.NB, NT = t.NT;
+    const int KR = (NB + 63) >> 6;
+    const int gw = blockIdx.x * Cf::WPB + wv, nwv = gridDim.x * Cf::WPB;
+    double *Wq = ws + (size_t)gw * wv_slot_doubles(NB);
+    double *Qn = Wq + (size_t)NB * SM;
+    double *Out = Qn + (size_t)NB * SM;
+    double *Rp = Out + (size_t)NB * 2 * SM;
+    double *Rq = Rp + SM * SM;
+    double *Ku = Rq + SM * SM;
+    const int cnt = uni(*wcount);
+    for (int idx = gw; idx < cnt; idx += nwv) {
+        const int e = uni(wlist[idx]);
+        int hi, lo;
+        tri_decode(e, hi, lo);
+        const int m = uni(k + 1 + hi), n = uni(k + 1 + lo);
+        const int ra = uni(*rank_at(t, m, k)), rc = uni(*rank_at(t, m, n)), rb = uni(*rank_at(t, n, k));
+        const int s = rc + ra;
+        const double *Au = lr_tile(t, m, k), *Av = Au + (size_t)NB * ra;
+        const double *Bu = lr_tile(t, n, k), *Bv = Bu + (size_t)NB * rb;
+        double *C = lr_tile(t, m, n);
+
+        // ---- X = Av^T Bv (ra x rb): lane owns a = (l&7)+8u, b = (l>>3)+8v.
+        //      Rows staged through LDS in chunks of xr (as large as fits) ----
+        {
+            const int la = lane & 7, lb = lane >> 3;
+            const int lx = (ra + rb) * 65 <= Cf::LDS ? 6 : ((ra + rb) * 33 <= Cf::LDS ? 5 : 4);
+            const int xr = 1 << lx, xld = xr + 1;
+            double xa[SM / 8][SM / 8];
+#pragma unroll
+            for (int u = 0; u < SM / 8; u++)
+#pragma unroll
+                for (int v = 0; v < SM / 8; v++) xa[u][v] = 0.0;
+            double *st = L;
+            for (int r0 = 0; r0 < NB; r0 += xr) {
+                const int nr = min(xr, NB - r0);
+#pragma unroll 4
+                for (int q = lane; q < xr * (ra + rb); q += 64) {
+                    const int rr = q & (xr - 1), c = q >> lx;
+                    double v = 0.0;
+                    if (rr < nr) v = (c < ra) ? Av[r0 + rr + (size_t)c * NB] : Bv[r0 + rr + (size_t)(c - ra) * NB];
+                    st[c * xld + rr] = v;
+                }
+                wv_sync();
+#pragma unroll 2
+                for (int rr = 0; rr < nr; rr++) {
+                    double av[SM / 8], bv[SM / 8];
+#pragma unroll
+                    for (int u = 0; u < SM / 8; u++) av[u] = (8 * u < ra) ? st[(la + 8 * u) * xld + rr] : 0.0;
+#pragma unroll
+                    for (int v = 0; v < SM / 8; v++) bv[v] = (8 * v < rb) ? st[(ra + lb + 8 * v) * xld + rr] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < SM / 8; u++)
+#pragma unroll
+                        for (int v = 0; v < SM / 8; v++)
+                            if (8 * u < ra && 8 * v < rb) xa[u][v] = fma(av[u], bv[v], xa[u][v]);
+                }
+                wv_sync();
+            }
+            // X^T into LDS as Xl[b * SM + a], zero for a >= ra
+            double *Xl = L;
+            for (int q = lane; q < rb * SM; q += 64) Xl[q] = 0.0;
+            wv_sync();
+#pragma unroll
+            for (int u = 0; u < SM / 8; u++)
+#pragma unroll
+                for (int v = 0; v < SM / 8; v++) {
+                    const int a = la + 8 * u, bb = lb + 8 * v;
+                    if (a < ra && bb < rb) Xl[bb * SM + a] = xa[u][v];
+                }
+            wv_sync();
+        }
+        // ---- Qn = -Bu X^T (NB x ra) ----
+        for (int g = 0; g < KR; g++) {
+            const int i = lane + 64 * g;
+            const bool ok = i < NB;
+            double ac[SM];
+#pragma unroll
+            for (int a = 0; a < SM; a++) ac[a] = 0.0;
+#pragma unroll 4
+            for (int bb = 0; bb < rb; bb++) {
+                const double bu = ok ? Bu[i + (size_t)bb * NB] : 0.0;
+                const double *xb = L + bb * SM;
+#pragma unroll
+                for (int a8 = 0; a8 < SM; a8 += 8)
+                    if (a8 < ra) {
+#pragma unroll
+                        for (int a = a8; a < a8 + 8; a++) ac[a] = fma(bu, xb[a], ac[a]);
+                    }
+            }
+            if (ok) {
+#pragma unroll
+                for (int a = 0; a < SM; a++)
+                    if (a < ra) Qn[i + (size_t)a * NB] = -ac[a];
+            }
+        }
+        wv_sync();
+        // ---- R factors of P = [Cu Au] and Q = [Cv Qn] ----
+        for (int side = 0; side < 2; side++) {
+            WvCols S{side ? C + (size_t)NB * rc : C, side ? (const double *)Qn : Au, rc, Wq, NB};
+            wv_geqrf_r<SM>(s, S, side ? Rq : Rp);
+        }
+        wv_sync();
+        // ---- rows of M = Rp Rq^T: lane c holds M(c, :) ----
+        double xm[SM];
+        {
+            for (int q = lane; q < s * SM; q += 64) {
+                const int l = q / SM, c = q - l * SM;
+                L[q] = (c < s) ? Rq[c * SM + l] : 0.0;  // L[l][c] = Rq(c, l)
+            }
+            wv_sync();
+#pragma unroll
+            for (int q = 0; q < SM; q++) xm[q] = 0.0;
+            for (int l = 0; l < s; l++) {
+                const double rp = (lane < s && l >= lane) ? Rp[lane * SM + l] : 0.0;
+                const double *rq = L + l * SM;
+#pragma unroll
+                for (int q8 = 0; q8 < SM; q8 += 8)
+                    if (q8 < s) {
+#pragma unroll
+                        for (int q = q8; q < q8 + 8; q++) xm[q] = fma(rp, rq[q], xm[q]);
+                    }
+            }
+            wv_sync();
+        }
+        // ---- one-sided Jacobi on the columns of M^T (lane c owns column c) ----
+        {
+            double *J = L;
+            const int npad = s + (s & 1);
+            const double tol = 4.0e-16 * sqrt((double)s);
+            double al = 0.0;
+#pragma unroll
+            for (int q = 0; q < SM; q++) al = fma(xm[q], xm[q], al);
+            // position of column `lane` in round 0 of the round-robin schedule
+            const int p0 = (lane == 0) ? 0 : lane;
+            for (int sweep = 0; sweep < 60; sweep++) {
+                int rot_any = 0;
+                int pos = p0;  // column at position p in round r: ((p - 1 + r) mod (npad - 1)) + 1
+                for (int r = 0; r < npad - 1; r++) {
+                    if (lane < s) {
+                        double *my = J + lane * Cf::LDJ;
+#pragma unroll
+                        for (int q8 = 0; q8 < SM; q8 += 8)
+                            if (q8 < s) {
+#pragma unroll
+                                for (int q = q8; q < q8 + 8; q++) my[q] = xm[q];
+                            }
+                        my[SM] = al;
+                    }
+                    wv_sync();
+                    const int pp = npad - 1 - pos;
+                    int pc = 0;
+                    if (pp != 0) {
+                        pc = pp + r;
+                        if (pc > npad - 1) pc -= npad - 1;
+                    }
+                    if (lane < s && pc < s) {
+                        const double *yp = J + pc * Cf::LDJ;
+                        const double be = yp[SM];
+                        double ga = 0.0;
+#pragma unroll
+                        for (int q8 = 0; q8 < SM; q8 += 8)
+                            if (q8 < s) {
+#pragma unroll
+                                for (int q = q8; q < q8 + 8; q++) ga = fma(xm[q], yp[q], ga);
+                            }
+                        const bool lo_ = lane < pc;
+                        double cs, sn;
+                        if (jacobi_rot(lo_ ? al : be, lo_ ? be : al, ga, tol, cs, sn)) {
+                            rot_any = 1;
+                            const double ss = lo_ ? -sn : sn;
+#pragma unroll
+                            for (int q8 = 0; q8 < SM; q8 += 8)
+                                if (q8 < s) {
+#pragma unroll
+                                    for (int q = q8; q < q8 + 8; q++) xm[q] = fma(ss, yp[q], cs * xm[q]);
+                                }
+                            // squared norm after the rotation
+                            al = lo_ ? (cs * cs * al - 2.0 * cs * sn * ga + sn * sn * be)
+                                     : (sn * sn * be + 2.0 * cs * sn * ga + cs * cs * al);
+                        }
+                    }
+                    // next round: every column but 0 moves one position down
+                    if (lane > 0) pos = (pos == 1) ? npad - 1 : pos - 1;
+                    wv_sync();
+                }
+                // exact norms once per sweep
+                al = 0.0;
+#pragma unroll
+                for (int q = 0; q < SM; q++) al = fma(xm[q], xm[q], al);
+                if (__ballot(rot_any) == 0) break;
+            }
+        }
+        // ---- singular values, order, rank ----
+        double sig = 0.0;
+#pragma unroll
+        for (int q = 0; q < SM; q++) sig = fma(xm[q], xm[q], sig);
+        sig = sqrt(sig);
+        if (lane < s) sg[lane] = sig;
+        wv_sync();
+        int pos = 0;
+        for (int q = 0; q < s; q++) {
+            const double o2 = sg[q];
+            pos += (o2 > sig) || (o2 == sig && q < lane);
+        }
+        int nr;
+        if (fixed_rk > 0) nr = fixed_rk < s ? fixed_rk : s;
+        else nr = __popcll(__ballot(lane < s && sig >= acc));
+        if (nr > maxrank) {
+            if (lane == 0) atomicAdd(ovf, 1);
+            nr = maxrank;
+        }
+        nr = uni(nr);
+        wv_sync();
+        if (nr > 0) {
+            // Vs_r into LDS, row-major: Vl[l * SM + pos]
+            double *Vl = L;
+            if (lane < s && pos < nr) {
+                const double inv = sig > 0.0 ? 1.0 / sig : 0.0;
+#pragma unroll
+                for (int q = 0; q < SM; q++)
+                    if (q < s) Vl[q * SM + pos] = xm[q] * inv;
+            }
+            wv_sync();
+            // Ku = Rq^T Vs_r: lane c computes row c -> Ku[c * SM + j]
+            {
+                double ac[SM];
+#pragma unroll
+                for (int j = 0; j < SM; j++) ac[j] = 0.0;
+                for (int l = 0; l < s; l++) {
+                    const double rq = (lane < s && l <= lane) ? Rq[l * SM + lane] : 0.0;
+                    const double *vr = Vl + l * SM;
+#pragma unroll
+                    for (int j8 = 0; j8 < SM; j8 += 8)
+                        if (j8 < nr) {
+#pragma unroll
+                            for (int j = j8; j < j8 + 8; j++) ac[j] = fma(rq, vr[j], ac[j]);
+                        }
+                }
+                if (lane < s) {
+#pragma unroll
+                    for (int j = 0; j < SM; j++)
+                        if (j < nr) Ku[lane * SM + j] = ac[j];
+                }
+            }
+            wv_sync();
+            // Kv = Rq^-1 Vs_r in place (lane j solves column j); pivots below
+            // 1e-14 of the largest give a zero component (exactly singular Rq)
+            double dmax = 0.0;
+            for (int q = 0; q < s; q++) dmax = fmax(dmax, fabs(Rq[q * SM + q]));
+            const double thr = 1e-14 * dmax;
+            if (lane < nr) {
+                for (int q = s - 1; q >= 0; q--) {
+                    double v = Vl[q * SM + lane];
+                    for (int l = q + 1; l < s; l++) v = fma(-Rq[q * SM + l], Vl[l * SM + lane], v);
+                    const double d = Rq[q * SM + q];
+                    Vl[q * SM + lane] = (fabs(d) > thr) ? v / d : 0.0;
+                }
+            }
+            wv_sync();
+            // ---- outputs.  V' = Q Kv first (Kv in LDS) into Out; then Ku is
+            //      staged into LDS and U' = P Ku is written straight over the
+            //      tile (rows are lane-local and Cu/Au of a row are read before
+            //      its U' row is written); finally V' moves next to U'. ----
+            for (int side = 1; side >= 0; side--) {
+                if (side == 0) {
+                    for (int q = lane; q < s * SM; q += 64) {
+                        const int c = q / SM, j = q - c * SM;
+                        Vl[q] = (j < nr) ? Ku[c * SM + j] : 0.0;
+                    }
+                    wv_sync();
+                }
+                double *dst = side ? Out + (size_t)nr * NB : C;
+                for (int g = 0; g < KR; g++) {
+                    const int i = lane + 64 * g;
+                    const bool ok = i < NB;
+                    double ac[SM];
+#pragma unroll
+                    for (int j = 0; j < SM; j++) ac[j] = 0.0;
+#pragma unroll 4
+                    for (int c = 0; c < s; c++) {
+                        double pv = 0.0;
+                        if (ok) {
+                            if (side == 0) pv = (c < rc) ? C[i + (size_t)c * NB] : Au[i + (size_t)(c - rc) * NB];
+                            else pv = (c < rc) ? C[i + (size_t)(rc + c) * NB] : Qn[i + (size_t)(c - rc) * NB];
+                        }
+                        const double *kr = Vl + c * SM;
+#pragma unroll
+                        for (int j8 = 0; j8 < SM; j8 += 8)
+                            if (j8 < nr) {
+#pragma unroll
+                                for (int j = j8; j < j8 + 8; j++) ac[j] = fma(pv, kr[j], ac[j]);
+                            }
+                    }
+                    if (ok) {
+#pragma unroll
+                        for (int j = 0; j < SM; j++)
+                            if (j < nr) dst[i + (size_t)j * NB] = ac[j];
+                    }
+                }
+                wv_sync();
+            }
+            // V' from Out into columns nr .. 2nr-1 of the tile (flat copy)
+            {
+                const double *src = Out + (size_t)nr * NB;
+                double *d2 = C + (size_t)nr * NB;
+                const int tot = nr * NB;
+                int q = lane;
+                for (; q + 64 * 7 < tot; q += 64 * 8) {
+                    double v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) v[u] = src[q + 64 * u];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) d2[q + 64 * u] = v[u];
+                }
+                for (; q < tot; q += 64) d2[q] = src[q];
+            }
+        }
+        if (lane == 0) {
+            *rank_at(t, m, n) = nr;
+            unsigned long long mm = tdim(t, m), ss = s;
+            unsigned long long qa = opc('q', mm, ss, 0, 0);
+            unsigned long long qb = opc('m', ra, rb, mm, 0) + opc('m', ra, mm, rb, 0) + opc('q', mm, ss, 0, 0);
+            unsigned long long sv = opc('r', ss, ss, 2, 0) + opc('s', ss, 0, 0, 0) + ss * nr;
+            unsigned long long nu = opc('o', mm, nr, ss, 1), nv = opc('o', nr, mm, ss, 2);
+            oplog(o, NT, qa + qb + sv + nu + nv, m, n, false, false);
+            atomicAdd(o.stat + 4, qa + qb + sv + nu + nv);
+            atomicAdd(o.stat + 5, 1ULL);
+            unsigned long long words = 2ULL * NB * (ra + rc + nr) + 2ULL * NB * rb;
+            atomicAdd(o.stat + 6, 8ULL * words);
+        }
+        wv_sync();
+    }
+}

@@ -238,8 +238,10 @@ __global__ void __launch_bounds__(BS) k_potrf_update(TLR t, int k, int j0, int j
 // and W = L^{-1} (ld PBK) to Winv[j0/PBK].  One workgroup of BS threads.
 #define PBK 64
 #define PLD (PBK + 1)
-__global__ void __launch_bounds__(BS) k_potrf_diag2(TLR t, int k, int j0, int jb, double *Winv, int *info)
+__global__ void __launch_bounds__(BS) k_potrf_diag2(TLR t, int k, int j0, int jb, double *Winv, int *info, OpLog o)
 {
+    // dpotrf of the whole tile counts once, on the band and on the critical path
+    if (j0 == 0 && threadIdx.x == 0) oplog(o, t.NT, opc('c', tdim(t, k), 0, 0, 0), k, k, true, true);
     __shared__ double L[PBK * PLD];
     __shared__ double W[PBK * PLD];
     __shared__ int bad;
