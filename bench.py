@@ -11,8 +11,9 @@ HBM: before every step the pristine compressed matrix is copied back
 (outside the timed region), then the factorization is timed between device
 synchronizations.
 
-Also reported: the roofline of the dominant kernel (the low-rank gemm +
-recompression launch, timed with HIP events on its own stream), and a CPU
+Also reported: the roofline of the dominant launch group (the low-rank gemm +
+recompression kernels of each panel step, timed with HIP events on the
+library's stream), and a CPU
 baseline: the oracle (C restatement of the reference, MKL sequential per task,
 OpenMP tasks on cores-1 threads) run on a bounded sample of the same matrix
 (the first panel steps), extrapolated to the whole factorization by the
@@ -178,7 +179,8 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": "k_lr_gemm (low-rank gemm + QR/Jacobi-SVD recompression, one launch per panel step)",
+            "kernel": ("recompression launch group of a panel step: k_classify + k_rc_wave<32|48|64> "
+                       "(one wave per low-rank gemm task: QR + Jacobi-SVD) + k_lr_gemm (stacks wider than 64)"),
             "achieved": round(achieved, 4),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",

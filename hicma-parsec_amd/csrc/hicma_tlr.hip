@@ -553,11 +553,11 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     // one-wave recompression (tasks with s <= WV_SMAX): 2 workgroups per CU of
     // the s <= 48 variant (4 waves each) and of the s <= 64 variant (2 waves)
     const bool wave = (NB <= 64 * WV_KMAX);
-    const int wgrid = wave ? 2 * ctx->ncu : 0;
+    const int wgrid = wave ? ctx->ncu : 0;  // times WvCfg<SM>::BPC resident workgroups
     const size_t wstride = wv_slot_doubles(NB);
     const int wcap = (int)(maxtask > 0 ? maxtask : 1);
-    size_t wws_d = (size_t)wgrid * WvCfg<48>::WPB * wstride;
-    size_t wlist_d = ((size_t)2 * wcap + 4) / 2 + 1;
+    size_t wws_d = (size_t)wgrid * 4 * wstride;  // four resident waves per CU
+    size_t wlist_d = ((size_t)3 * wcap + 8) / 2 + 1;
     size_t total = g.stride * ns + syrk_d + lrd_d + fstride * nsf + tasks_d + winv_d + wws_d + wlist_d;
     int rc = ensure_ws(ctx, total * sizeof(double));
     if (rc) return rc;
@@ -571,14 +571,17 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     int *tasks_s = (int *)(Winv + winv_d);
     double *wws = Winv + winv_d + tasks_d;
     int *wcount = (int *)(wws + wws_d);
-    int *wlist = wcount + 4;
+    int *wlist = wcount + 8;  // counts [0..2], next-task counters [3..5]
     if (fast)
         HIPCHK(hipFuncSetAttribute((const void *)k_lr_gemm_fast, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(RCF_LDS * sizeof(double))));
     HIPCHK(hipFuncSetAttribute((const void *)k_lr_gemm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rc_lds));
+    const size_t wlds32 = sizeof(double) * WvCfg<32>::LDS * WvCfg<32>::WPB;
     const size_t wlds48 = sizeof(double) * WvCfg<48>::LDS * WvCfg<48>::WPB;
     const size_t wlds64 = sizeof(double) * WvCfg<64>::LDS * WvCfg<64>::WPB;
     if (wave) {
+        HIPCHK(hipFuncSetAttribute((const void *)k_rc_wave<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)wlds32));
         HIPCHK(hipFuncSetAttribute((const void *)k_rc_wave<48>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)wlds48));
         HIPCHK(hipFuncSetAttribute((const void *)k_rc_wave<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -633,17 +636,21 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
             }
             hipLaunchKernelGGL(k_classify, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k, tasks_s);
             if (wave) {
-                HIPCHK(hipMemsetAsync(wcount, 0, 2 * sizeof(int), st));
+                HIPCHK(hipMemsetAsync(wcount, 0, 6 * sizeof(int), st));
                 hipLaunchKernelGGL(k_classify_wave, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k,
                                    tasks_s, wlist, wcount, wcap);
-                int wg = (int)std::min<long long>(wgrid, (ntask + WvCfg<48>::WPB - 1) / WvCfg<48>::WPB);
-                hipLaunchKernelGGL(k_rc_wave<48>, dim3(wg), dim3(64 * WvCfg<48>::WPB), wlds48, st, t, k,
-                                   (const int *)wlist, (const int *)wcount, wws, acc, fixed_rank, maxrank, o,
+                int wg = (int)std::min<long long>(wgrid * WvCfg<32>::BPC, (ntask + WvCfg<32>::WPB - 1) / WvCfg<32>::WPB);
+                hipLaunchKernelGGL(k_rc_wave<32>, dim3(wg), dim3(64 * WvCfg<32>::WPB), wlds32, st, t, k,
+                                   (const int *)wlist, (const int *)wcount, wcount + 3, wws, acc, fixed_rank, maxrank, o,
                                    ctx->d_ovf);
-                wg = (int)std::min<long long>(wgrid, (ntask + WvCfg<64>::WPB - 1) / WvCfg<64>::WPB);
-                hipLaunchKernelGGL(k_rc_wave<64>, dim3(wg), dim3(64 * WvCfg<64>::WPB), wlds64, st, t, k,
-                                   (const int *)(wlist + wcap), (const int *)(wcount + 1), wws, acc, fixed_rank,
+                wg = (int)std::min<long long>(wgrid * WvCfg<48>::BPC, (ntask + WvCfg<48>::WPB - 1) / WvCfg<48>::WPB);
+                hipLaunchKernelGGL(k_rc_wave<48>, dim3(wg), dim3(64 * WvCfg<48>::WPB), wlds48, st, t, k,
+                                   (const int *)(wlist + wcap), (const int *)(wcount + 1), wcount + 4, wws, acc, fixed_rank,
                                    maxrank, o, ctx->d_ovf);
+                wg = (int)std::min<long long>(wgrid * WvCfg<64>::BPC, (ntask + WvCfg<64>::WPB - 1) / WvCfg<64>::WPB);
+                hipLaunchKernelGGL(k_rc_wave<64>, dim3(wg), dim3(64 * WvCfg<64>::WPB), wlds64, st, t, k,
+                                   (const int *)(wlist + 2 * (size_t)wcap), (const int *)(wcount + 2), wcount + 5, wws, acc,
+                                   fixed_rank, maxrank, o, ctx->d_ovf);
             }
             if (fast) {
                 int gf = (int)std::min<long long>(nsf, ntask);

@@ -26,14 +26,15 @@
 #define WV_KMAX 16   // rows per lane: NB <= 64 * WV_KMAX
 #define WV_PW 4      // Householder panel width
 
-// Per-variant sizes: SM = largest stack width of the variant (48 or 64).
+// Per-variant sizes: SM = largest stack width of the variant (32, 48 or 64).
 template <int SM>
 struct WvCfg {
     static constexpr int LDJ = SM + 2;             // Jacobi column stride in LDS
     static constexpr int SIG = SM * LDJ;           // offset of the singular values
     static constexpr int LDS = SIG + 2 * SM;       // doubles of LDS per wave
     static constexpr int WPB = SM <= 48 ? 4 : 2;   // waves (tasks in flight) per workgroup
-    static constexpr int MINW = SM <= 48 ? 2 : 1;  // waves per SIMD the registers must allow
+    static constexpr int MINW = 1;                 // one wave per SIMD: the full register file
+    static constexpr int BPC = 4 / WPB;            // resident workgroups per CU
 };
 
 // per-wave HBM workspace: W (QR copy), Qn, Out (2 blocks), Rp, Rq, Ku
@@ -59,6 +60,64 @@ __device__ __forceinline__ double rdlane(double v, int l)
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// FP64 matrix core tile C += A B with A 16x4, B 4x16, C 16x16 (gfx950
+// V_MFMA_F64_16X16X4_F64).  Operand layout on the wave: A[i][k] on lane
+// i + 16k, B[k][j] on lane j + 16k; result C[i][j] on lane j + 16 (i & 3),
+// register i >> 2.
+typedef double wv_d4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ wv_d4 mfma16(double a, double b, wv_d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Tall product D (NB x nc, ld NB) = alpha * A (NB x kd) K (kd x nc) with A's
+// column c at acol(c) and K in LDS (K[c][j] at Kl[c * ldk + j], zero for
+// j >= nc).  One 16-row tile at a time, all its A loads issued up front;
+// the K operands are read from LDS.  Rows of a tile are read before any of
+// them is written, so D may overwrite A's storage row for row.
+template <int SM, typename ACol>
+__device__ __forceinline__ void wv_tall_mm(int NB, int kd, int nc, double alpha, ACol acol, const double *Kl,
+                                           int ldk, double *D)
+{
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+    constexpr int KS = SM / 4, TN = SM / 16;
+#pragma unroll 1
+    for (int r0 = 0; r0 < NB; r0 += 16) {
+        const int row = r0 + i16;
+        double av[KS];
+#pragma unroll
+        for (int kk = 0; kk < KS; kk++) {
+            const int c = 4 * kk + k4;
+            av[kk] = (4 * kk < kd && c < kd && row < NB) ? acol(c)[row] : 0.0;
+        }
+        wv_d4 acc[TN];
+#pragma unroll
+        for (int tn = 0; tn < TN; tn++) acc[tn] = wv_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < KS; kk++)
+            if (4 * kk < kd) {
+                const int kr = 4 * kk + k4;
+#pragma unroll
+                for (int tn = 0; tn < TN; tn++)
+                    if (16 * tn < nc) {
+                        const double kb = (kr < kd) ? Kl[kr * ldk + 16 * tn + i16] : 0.0;
+                        acc[tn] = mfma16(av[kk], kb, acc[tn]);
+                    }
+            }
+#pragma unroll
+        for (int tn = 0; tn < TN; tn++) {
+            if (16 * tn < nc) {
+                const int j = 16 * tn + i16;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = r0 + k4 + 4 * r;
+                    if (j < nc && i < NB) D[i + (size_t)j * NB] = alpha * acc[tn][r];
+                }
+            }
+        }
+    }
+}
+
 // reduce-scatter of 4 values: lane l returns the wave total of value l >> 4
 __device__ __forceinline__ double wrs4(double p0, double p1, double p2, double p3)
 {
@@ -68,6 +127,29 @@ __device__ __forceinline__ double wrs4(double p0, double p1, double p2, double p
     double a = x0 + y0, b = x1 + y1;
     swap16d(a, b);    // value index = lane >> 4
     return gsum<16>(a + b);
+}
+
+// reduce-scatter of 8 values: lane l returns the wave total of value l >> 3
+__device__ __forceinline__ double wrs8(double (&p)[8])
+{
+    const int lane = threadIdx.x & 63;
+    double q[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double x = p[k], y = p[k + 4];
+        swap32d(x, y);  // lanes < 32: values 0..3, lanes >= 32: values 4..7
+        q[k] = x + y;
+    }
+    double r[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        double x = q[k], y = q[k + 2];
+        swap16d(x, y);  // value bit 1 = lane bit 4
+        r[k] = x + y;
+    }
+    const bool up = (lane & 8) != 0;
+    const double keep = up ? r[1] : r[0], send = up ? r[0] : r[1];
+    return gsum<8>(keep + dppd<DPP_ROR8>(send));  // value bit 0 = lane bit 3
 }
 
 // Column c of the stack: first pass from the tiles, afterwards from W.
@@ -176,52 +258,79 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
 #pragma unroll
         for (int c = 0; c < WV_PW; c++)
             v0[c] = (c < jb) ? ((lane == j0 + c) ? 1.0 : (lane > j0 + c ? a[0][c] : 0.0)) : 0.0;
-        // ---- trailing columns: b -= V (T^T (V^T b)) ----
-        for (int t = j0 + jb; t < s; t++) {
-            const double *col = A.col(t, first);
-            double b[WV_KMAX];
+        // ---- trailing columns, two at a time: b -= V (T^T (V^T b)) ----
+        for (int t = j0 + jb; t < s; t += 2) {
+            const bool two = (t + 1 < s);
+            const double *col0 = A.col(t, first), *col1 = A.col(two ? t + 1 : t, first);
+            double b0[WV_KMAX], b1[WV_KMAX];
 #pragma unroll
             for (int k = 0; k < WV_KMAX; k++) {
                 const int i = lane + 64 * k;
-                b[k] = (i < NB) ? col[i] : 0.0;
+                b0[k] = (i < NB) ? col0[i] : 0.0;
+                b1[k] = (two && i < NB) ? col1[i] : 0.0;
             }
-            double p[WV_PW];
+            double p[8];
 #pragma unroll
             for (int c = 0; c < WV_PW; c++) {
-                double acc = v0[c] * b[0];
+                double a0 = v0[c] * b0[0], a1 = v0[c] * b1[0];
 #pragma unroll
-                for (int k = 1; k < WV_KMAX; k++) acc = fma(a[k][c], b[k], acc);
-                p[c] = acc;
+                for (int k = 1; k < WV_KMAX; k++) {
+                    a0 = fma(a[k][c], b0[k], a0);
+                    a1 = fma(a[k][c], b1[k], a1);
+                }
+                p[c] = a0;
+                p[c + 4] = a1;
             }
-            const double red = wrs4(p[0], p[1], p[2], p[3]);
-            double d[WV_PW], w[WV_PW];
+            const double red = wrs8(p);
+            double w0[WV_PW], w1[WV_PW];
+            {
+                double d0[WV_PW], d1[WV_PW];
 #pragma unroll
-            for (int c = 0; c < WV_PW; c++) d[c] = rdlane(red, 16 * c);
+                for (int c = 0; c < WV_PW; c++) {
+                    d0[c] = rdlane(red, 8 * c);
+                    d1[c] = rdlane(red, 8 * (c + 4));
+                }
 #pragma unroll
-            for (int c = 0; c < WV_PW; c++) {
-                double acc = 0.0;
+                for (int c = 0; c < WV_PW; c++) {
+                    double x0 = 0.0, x1 = 0.0;
 #pragma unroll
-                for (int q = 0; q <= c; q++) acc = fma(T[q][c], d[q], acc);
-                w[c] = acc;
+                    for (int q = 0; q <= c; q++) {
+                        x0 = fma(T[q][c], d0[q], x0);
+                        x1 = fma(T[q][c], d1[q], x1);
+                    }
+                    w0[c] = x0;
+                    w1[c] = x1;
+                }
             }
             {
-                double acc = b[0];
+                double x0 = b0[0], x1 = b1[0];
 #pragma unroll
-                for (int c = 0; c < WV_PW; c++) acc = fma(-v0[c], w[c], acc);
-                b[0] = acc;
+                for (int c = 0; c < WV_PW; c++) {
+                    x0 = fma(-v0[c], w0[c], x0);
+                    x1 = fma(-v0[c], w1[c], x1);
+                }
+                b0[0] = x0;
+                b1[0] = x1;
             }
 #pragma unroll
             for (int k = 1; k < WV_KMAX; k++) {
-                double acc = b[k];
+                double x0 = b0[k], x1 = b1[k];
 #pragma unroll
-                for (int c = 0; c < WV_PW; c++) acc = fma(-a[k][c], w[c], acc);
-                b[k] = acc;
+                for (int c = 0; c < WV_PW; c++) {
+                    x0 = fma(-a[k][c], w0[c], x0);
+                    x1 = fma(-a[k][c], w1[c], x1);
+                }
+                b0[k] = x0;
+                b1[k] = x1;
             }
-            double *out = A.W + (size_t)t * NB;
+            double *out0 = A.W + (size_t)t * NB, *out1 = A.W + (size_t)(t + 1) * NB;
 #pragma unroll
             for (int k = 0; k < WV_KMAX; k++) {
                 const int i = lane + 64 * k;
-                if (i < NB) out[i] = b[k];
+                if (i < NB) {
+                    out0[i] = b0[k];
+                    if (two) out1[i] = b1[k];
+                }
             }
         }
     }
@@ -229,7 +338,7 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
 
 // Tasks of step k whose update runs on one wave: both factors low rank,
 // s = rc + ra in [1, WV_SMAX], rb <= WV_SMAX, NB <= 64 * WV_KMAX.  They are
-// appended to list 0 (s, rb <= 48) or list 1 (up to WV_SMAX); the order inside
+// appended to list 0 (s, rb <= 32), 1 (<= 48) or 2 (<= WV_SMAX); the order inside
 // a list is irrelevant (tasks of one step are independent).  Listed tasks are
 // marked -1 in tasks_s; every other task keeps its k_classify value.
 __global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wcount, int cap)
@@ -246,15 +355,15 @@ __global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wco
     const int s = rc + ra;
     if (ra > 0 && rb > 0 && s <= WV_SMAX && rb <= WV_SMAX) {
         tasks_s[e] = -1;
-        const int l = (s <= 48 && rb <= 48) ? 0 : 1;
+        const int l = (s <= 32 && rb <= 32) ? 0 : ((s <= 48 && rb <= 48) ? 1 : 2);
         wlist[(size_t)l * cap + atomicAdd(wcount + l, 1)] = (int)e;
     }
 }
 
 template <int SM>
 __global__ void __launch_bounds__(64 * WvCfg<SM>::WPB, WvCfg<SM>::MINW)
-k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, double *ws, double acc, int fixed_rk, int maxrank,
-          OpLog o, int *ovf)
+k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double *ws, double acc, int fixed_rk,
+          int maxrank, OpLog o, int *ovf)
 {
     using Cf = WvCfg<SM>;
     extern __shared__ double smem[];
@@ -264,7 +373,7 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, double *ws, double 
     double *sg = L + Cf::SIG;
     const int NB = t.NB, NT = t.NT;
     const int KR = (NB + 63) >> 6;
-    const int gw = blockIdx.x * Cf::WPB + wv, nwv = gridDim.x * Cf::WPB;
+    const int gw = blockIdx.x * Cf::WPB + wv;
     double *Wq = ws + (size_t)gw * wv_slot_doubles(NB);
     double *Qn = Wq + (size_t)NB * SM;
     double *Out = Qn + (size_t)NB * SM;
@@ -272,7 +381,12 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, double *ws, double 
     double *Rq = Rp + SM * SM;
     double *Ku = Rq + SM * SM;
     const int cnt = uni(*wcount);
-    for (int idx = gw; idx < cnt; idx += nwv) {
+    // tasks are taken one at a time from a shared counter (costs differ)
+    for (;;) {
+        int idx = 0;
+        if (lane == 0) idx = atomicAdd(wnext, 1);
+        idx = uni(idx);
+        if (idx >= cnt) break;
         const int e = uni(wlist[idx]);
         int hi, lo;
         tri_decode(e, hi, lo);
@@ -283,80 +397,57 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, double *ws, double 
         const double *Bu = lr_tile(t, n, k), *Bv = Bu + (size_t)NB * rb;
         double *C = lr_tile(t, m, n);
 
-        // ---- X = Av^T Bv (ra x rb): lane owns a = (l&7)+8u, b = (l>>3)+8v.
-        //      Rows staged through LDS in chunks of xr (as large as fits) ----
+        // ---- X = Av^T Bv (ra x rb) on the matrix cores: 16x16 tiles of X,
+        //      k runs over the NB rows four at a time ----
         {
-            const int la = lane & 7, lb = lane >> 3;
-            const int lx = (ra + rb) * 65 <= Cf::LDS ? 6 : ((ra + rb) * 33 <= Cf::LDS ? 5 : 4);
-            const int xr = 1 << lx, xld = xr + 1;
-            double xa[SM / 8][SM / 8];
+            const int i16 = lane & 15, k4 = lane >> 4;
+            constexpr int TT = SM / 16, U = 4;
+            wv_d4 xacc[TT][TT];
 #pragma unroll
-            for (int u = 0; u < SM / 8; u++)
+            for (int ta = 0; ta < TT; ta++)
 #pragma unroll
-                for (int v = 0; v < SM / 8; v++) xa[u][v] = 0.0;
-            double *st = L;
-            for (int r0 = 0; r0 < NB; r0 += xr) {
-                const int nr = min(xr, NB - r0);
-#pragma unroll 4
-                for (int q = lane; q < xr * (ra + rb); q += 64) {
-                    const int rr = q & (xr - 1), c = q >> lx;
-                    double v = 0.0;
-                    if (rr < nr) v = (c < ra) ? Av[r0 + rr + (size_t)c * NB] : Bv[r0 + rr + (size_t)(c - ra) * NB];
-                    st[c * xld + rr] = v;
-                }
-                wv_sync();
-#pragma unroll 2
-                for (int rr = 0; rr < nr; rr++) {
-                    double av[SM / 8], bv[SM / 8];
+                for (int tb = 0; tb < TT; tb++) xacc[ta][tb] = wv_d4{0.0, 0.0, 0.0, 0.0};
+            for (int r0 = 0; r0 < NB; r0 += 4 * U) {
+                double av[U][TT], bv[U][TT];
 #pragma unroll
-                    for (int u = 0; u < SM / 8; u++) av[u] = (8 * u < ra) ? st[(la + 8 * u) * xld + rr] : 0.0;
+                for (int u = 0; u < U; u++) {
+                    const int row = r0 + 4 * u + k4;
 #pragma unroll
-                    for (int v = 0; v < SM / 8; v++) bv[v] = (8 * v < rb) ? st[(ra + lb + 8 * v) * xld + rr] : 0.0;
-#pragma unroll
-                    for (int u = 0; u < SM / 8; u++)
-#pragma unroll
-                        for (int v = 0; v < SM / 8; v++)
-                            if (8 * u < ra && 8 * v < rb) xa[u][v] = fma(av[u], bv[v], xa[u][v]);
-                }
-                wv_sync();
-            }
-            // X^T into LDS as Xl[b * SM + a], zero for a >= ra
-            double *Xl = L;
-            for (int q = lane; q < rb * SM; q += 64) Xl[q] = 0.0;
-            wv_sync();
-#pragma unroll
-            for (int u = 0; u < SM / 8; u++)
-#pragma unroll
-                for (int v = 0; v < SM / 8; v++) {
-                    const int a = la + 8 * u, bb = lb + 8 * v;
-                    if (a < ra && bb < rb) Xl[bb * SM + a] = xa[u][v];
-                }
-            wv_sync();
-        }
-        // ---- Qn = -Bu X^T (NB x ra) ----
-        for (int g = 0; g < KR; g++) {
-            const int i = lane + 64 * g;
-            const bool ok = i < NB;
-            double ac[SM];
-#pragma unroll
-            for (int a = 0; a < SM; a++) ac[a] = 0.0;
-#pragma unroll 4
-            for (int bb = 0; bb < rb; bb++) {
-                const double bu = ok ? Bu[i + (size_t)bb * NB] : 0.0;
-                const double *xb = L + bb * SM;
-#pragma unroll
-                for (int a8 = 0; a8 < SM; a8 += 8)
-                    if (a8 < ra) {
-#pragma unroll
-                        for (int a = a8; a < a8 + 8; a++) ac[a] = fma(bu, xb[a], ac[a]);
+                    for (int tt = 0; tt < TT; tt++) {
+                        const int a = 16 * tt + i16;
+                        av[u][tt] = (16 * tt < ra && a < ra && row < NB) ? Av[row + (size_t)a * NB] : 0.0;
+                        bv[u][tt] = (16 * tt < rb && a < rb && row < NB) ? Bv[row + (size_t)a * NB] : 0.0;
                     }
-            }
-            if (ok) {
+                }
 #pragma unroll
-                for (int a = 0; a < SM; a++)
-                    if (a < ra) Qn[i + (size_t)a * NB] = -ac[a];
+                for (int u = 0; u < U; u++)
+#pragma unroll
+                    for (int ta = 0; ta < TT; ta++)
+                        if (16 * ta < ra) {
+#pragma unroll
+                            for (int tb = 0; tb < TT; tb++)
+                                if (16 * tb < rb) xacc[ta][tb] = mfma16(av[u][ta], bv[u][tb], xacc[ta][tb]);
+                        }
             }
+            // X^T into LDS as Xl[b * SM + a], zero outside ra x rb
+            double *Xl = L;
+            for (int q = lane; q < SM * SM; q += 64) Xl[q] = 0.0;
+            wv_sync();
+#pragma unroll
+            for (int ta = 0; ta < TT; ta++)
+#pragma unroll
+                for (int tb = 0; tb < TT; tb++)
+                    if (16 * ta < ra && 16 * tb < rb) {
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int a = 16 * ta + k4 + 4 * r, bb = 16 * tb + i16;
+                            if (a < ra && bb < rb) Xl[bb * SM + a] = xacc[ta][tb][r];
+                        }
+                    }
+            wv_sync();
         }
+        // ---- Qn = -Bu X^T (NB x ra): K = X^T, K[b][a] = Xl[b * SM + a] ----
+        wv_tall_mm<SM>(NB, rb, ra, -1.0, [&](int c) { return Bu + (size_t)c * NB; }, L, SM, Qn);
         wv_sync();
         // ---- R factors of P = [Cu Au] and Q = [Cv Qn] ----
         for (int side = 0; side < 2; side++) {
@@ -394,6 +485,10 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, double *ws, double 
             double al = 0.0;
 #pragma unroll
             for (int q = 0; q < SM; q++) al = fma(xm[q], xm[q], al);
+            // rounding-noise floor of the squared column norms: a pair whose
+            // smaller column is below it is left alone (its rotation changes
+            // the larger column by less than one rounding)
+            const double nz = (2.3e-16 * s) * (2.3e-16 * s) * wsum(lane < s ? al : 0.0);
             // position of column `lane` in round 0 of the round-robin schedule
             const int p0 = (lane == 0) ? 0 : lane;
             for (int sweep = 0; sweep < 60; sweep++) {
@@ -420,16 +515,17 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, double *ws, double 
                     if (lane < s && pc < s) {
                         const double *yp = J + pc * Cf::LDJ;
                         const double be = yp[SM];
-                        double ga = 0.0;
+                        double g4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                         for (int q8 = 0; q8 < SM; q8 += 8)
                             if (q8 < s) {
 #pragma unroll
-                                for (int q = q8; q < q8 + 8; q++) ga = fma(xm[q], yp[q], ga);
+                                for (int q = q8; q < q8 + 8; q++) g4[q & 3] = fma(xm[q], yp[q], g4[q & 3]);
                             }
+                        const double ga = (g4[0] + g4[1]) + (g4[2] + g4[3]);
                         const bool lo_ = lane < pc;
                         double cs, sn;
-                        if (jacobi_rot(lo_ ? al : be, lo_ ? be : al, ga, tol, cs, sn)) {
+                        if (fmin(al, be) > nz && jacobi_rot(lo_ ? al : be, lo_ ? be : al, ga, tol, cs, sn)) {
                             rot_any = 1;
                             const double ss = lo_ ? -sn : sn;
 #pragma unroll
@@ -523,46 +619,21 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, double *ws, double 
             wv_sync();
             // ---- outputs.  V' = Q Kv first (Kv in LDS) into Out; then Ku is
             //      staged into LDS and U' = P Ku is written straight over the
-            //      tile (rows are lane-local and Cu/Au of a row are read before
-            //      its U' row is written); finally V' moves next to U'. ----
-            for (int side = 1; side >= 0; side--) {
-                if (side == 0) {
-                    for (int q = lane; q < s * SM; q += 64) {
-                        const int c = q / SM, j = q - c * SM;
-                        Vl[q] = (j < nr) ? Ku[c * SM + j] : 0.0;
-                    }
-                    wv_sync();
-                }
-                double *dst = side ? Out + (size_t)nr * NB : C;
-                for (int g = 0; g < KR; g++) {
-                    const int i = lane + 64 * g;
-                    const bool ok = i < NB;
-                    double ac[SM];
-#pragma unroll
-                    for (int j = 0; j < SM; j++) ac[j] = 0.0;
-#pragma unroll 4
-                    for (int c = 0; c < s; c++) {
-                        double pv = 0.0;
-                        if (ok) {
-                            if (side == 0) pv = (c < rc) ? C[i + (size_t)c * NB] : Au[i + (size_t)(c - rc) * NB];
-                            else pv = (c < rc) ? C[i + (size_t)(rc + c) * NB] : Qn[i + (size_t)(c - rc) * NB];
-                        }
-                        const double *kr = Vl + c * SM;
-#pragma unroll
-                        for (int j8 = 0; j8 < SM; j8 += 8)
-                            if (j8 < nr) {
-#pragma unroll
-                                for (int j = j8; j < j8 + 8; j++) ac[j] = fma(pv, kr[j], ac[j]);
-                            }
-                    }
-                    if (ok) {
-#pragma unroll
-                        for (int j = 0; j < SM; j++)
-                            if (j < nr) dst[i + (size_t)j * NB] = ac[j];
-                    }
-                }
-                wv_sync();
+            //      tile (a 16-row tile of P is read before its U' rows are
+            //      written); finally V' moves next to U'. ----
+            wv_tall_mm<SM>(NB, s, nr, 1.0,
+                           [&](int c) { return c < rc ? C + (size_t)(rc + c) * NB : Qn + (size_t)(c - rc) * NB; },
+                           Vl, SM, Out + (size_t)nr * NB);
+            wv_sync();
+            for (int q = lane; q < s * SM; q += 64) {
+                const int c = q / SM, j = q - c * SM;
+                Vl[q] = (j < nr) ? Ku[c * SM + j] : 0.0;
             }
+            wv_sync();
+            wv_tall_mm<SM>(NB, s, nr, 1.0,
+                           [&](int c) { return c < rc ? (const double *)C + (size_t)c * NB : Au + (size_t)(c - rc) * NB; },
+                           Vl, SM, C);
+            wv_sync();
             // V' from Out into columns nr .. 2nr-1 of the tile (flat copy)
             {
                 const double *src = Out + (size_t)nr * NB;
