@@ -37,6 +37,10 @@ struct hicma_ctx {
     size_t lds_max;
     int profile;                     // record events around recompression launches
     std::vector<hipEvent_t> prof_ev; // pairs
+    // the recompression variants of one panel step run concurrently on side
+    // streams (default priority), forked from and joined back to `stream`
+    hipStream_t side[3];
+    hipEvent_t ev_fork, ev_join[3];
 };
 
 struct hicma_tlr_desc {
@@ -154,6 +158,11 @@ extern "C" int hicma_init(hicma_ctx **ctx, int device)
     c->ncu = prop.multiProcessorCount;
     c->lds_max = prop.sharedMemPerBlock;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int i = 0; i < 3; i++) {
+        HIPCHK(hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     HIPCHK(hipMalloc(&c->d_info, 4 * sizeof(int)));
     c->d_ovf = c->d_info + 1;
     HIPCHK(hipMalloc(&c->d_scalar, 8 * sizeof(double)));
@@ -169,6 +178,11 @@ extern "C" int hicma_fini(hicma_ctx *c)
     hipFree(c->d_info);
     hipFree(c->d_scalar);
     for (auto e : c->prof_ev) (void)hipEventDestroy(e);
+    for (int i = 0; i < 3; i++) {
+        (void)hipEventDestroy(c->ev_join[i]);
+        (void)hipStreamDestroy(c->side[i]);
+    }
+    (void)hipEventDestroy(c->ev_fork);
     hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -483,7 +497,7 @@ static int launch_potrf(hicma_tlr_desc *A, int k, double *Winv, OpLog o)
     hipStream_t s = A->ctx->stream;
     for (int j0 = 0; j0 < t.NB; j0 += PBK) {
         int jb = std::min(PBK, t.NB - j0);
-        hipLaunchKernelGGL(k_potrf_diag2, dim3(1), dim3(BS), 0, s, t, k, j0, jb, Winv, A->ctx->d_info, o);
+        hipLaunchKernelGGL(k_potrf_diag3, dim3(1), dim3(64), 0, s, t, k, j0, jb, Winv, A->ctx->d_info, o);
         int R = t.NB - j0 - jb;
         if (R > 0) {
             hipLaunchKernelGGL(k_potrf_panel2, dim3((R + PBK - 1) / PBK), dim3(BS), 0, s, t, k, j0, jb,
@@ -556,9 +570,12 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     const int wgrid = wave ? ctx->ncu : 0;  // times WvCfg<SM>::BPC resident workgroups
     const size_t wstride = wv_slot_doubles(NB);
     const int wcap = (int)(maxtask > 0 ? maxtask : 1);
-    size_t wws_d = (size_t)wgrid * 4 * wstride;  // four resident waves per CU
-    size_t wlist_d = ((size_t)3 * wcap + 8) / 2 + 1;
-    size_t total = g.stride * ns + syrk_d + lrd_d + fstride * nsf + tasks_d + winv_d + wws_d + wlist_d;
+    // four concurrent variants (32, 48, 64 on side streams), each with up to
+    // four resident waves per CU: one quarter of the region each
+    size_t wws_d = (size_t)4 * wgrid * 4 * wstride;
+    size_t wbs_d = wave ? (size_t)wgrid * wb_slot_doubles(NB) : 0;  // one wide-stack wave per CU
+    size_t wlist_d = ((size_t)4 * wcap + 8) / 2 + 1;
+    size_t total = g.stride * ns + syrk_d + lrd_d + fstride * nsf + tasks_d + winv_d + wws_d + wlist_d + wbs_d;
     int rc = ensure_ws(ctx, total * sizeof(double));
     if (rc) return rc;
     double *rcws = ctx->ws;
@@ -571,7 +588,8 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     int *tasks_s = (int *)(Winv + winv_d);
     double *wws = Winv + winv_d + tasks_d;
     int *wcount = (int *)(wws + wws_d);
-    int *wlist = wcount + 8;  // counts [0..2], next-task counters [3..5]
+    int *wlist = wcount + 8;  // counts [0..3], next-task counters [4..7]
+    double *wbs = (double *)wcount + wlist_d;
     if (fast)
         HIPCHK(hipFuncSetAttribute((const void *)k_lr_gemm_fast, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(RCF_LDS * sizeof(double))));
@@ -586,6 +604,8 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
                                    (int)wlds48));
         HIPCHK(hipFuncSetAttribute((const void *)k_rc_wave<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)wlds64));
+        HIPCHK(hipFuncSetAttribute((const void *)k_rc_wide, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(sizeof(double) * WB_LDS)));
     }
 
     std::vector<hipEvent_t> ev;
@@ -608,7 +628,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
         if (nbnd > 0)
             hipLaunchKernelGGL(k_trsm_band, dim3(ntg, nbnd), dim3(BS), 0, st, t, k, o);
         if (nlr > 0)
-            hipLaunchKernelGGL(k_trsm_lr2, dim3((cap + 15) / 16, nlr), dim3(BS), 0, st, t, k, (const double *)Winv, o);
+            hipLaunchKernelGGL(k_trsm_lr3, dim3((cap + 15) / 16, nlr), dim3(BS), 0, st, t, k, (const double *)Winv, o);
         mark();
         if (nbnd > 0) hipLaunchKernelGGL(k_syrk_band, dim3(ntl, nbnd), dim3(BS), 0, st, t, k, o);
         if (nlr > 0) {
@@ -636,21 +656,33 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
             }
             hipLaunchKernelGGL(k_classify, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k, tasks_s);
             if (wave) {
-                HIPCHK(hipMemsetAsync(wcount, 0, 6 * sizeof(int), st));
+                HIPCHK(hipMemsetAsync(wcount, 0, 8 * sizeof(int), st));
                 hipLaunchKernelGGL(k_classify_wave, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k,
                                    tasks_s, wlist, wcount, wcap);
+                // fork: variants 48, 64 and the wide one on the side streams
+                HIPCHK(hipEventRecord(ctx->ev_fork, st));
+                for (int i = 0; i < 3; i++) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->ev_fork, 0));
                 int wg = (int)std::min<long long>(wgrid * WvCfg<32>::BPC, (ntask + WvCfg<32>::WPB - 1) / WvCfg<32>::WPB);
                 hipLaunchKernelGGL(k_rc_wave<32>, dim3(wg), dim3(64 * WvCfg<32>::WPB), wlds32, st, t, k,
-                                   (const int *)wlist, (const int *)wcount, wcount + 3, wws, acc, fixed_rank, maxrank, o,
-                                   ctx->d_ovf);
+                                   (const int *)wlist, (const int *)wcount, wcount + 4, wws, acc, fixed_rank, maxrank,
+                                   o, ctx->d_ovf);
                 wg = (int)std::min<long long>(wgrid * WvCfg<48>::BPC, (ntask + WvCfg<48>::WPB - 1) / WvCfg<48>::WPB);
-                hipLaunchKernelGGL(k_rc_wave<48>, dim3(wg), dim3(64 * WvCfg<48>::WPB), wlds48, st, t, k,
-                                   (const int *)(wlist + wcap), (const int *)(wcount + 1), wcount + 4, wws, acc, fixed_rank,
-                                   maxrank, o, ctx->d_ovf);
+                hipLaunchKernelGGL(k_rc_wave<48>, dim3(wg), dim3(64 * WvCfg<48>::WPB), wlds48, ctx->side[0], t, k,
+                                   (const int *)(wlist + wcap), (const int *)(wcount + 1), wcount + 5, wws + wws_d / 4,
+                                   acc, fixed_rank, maxrank, o, ctx->d_ovf);
                 wg = (int)std::min<long long>(wgrid * WvCfg<64>::BPC, (ntask + WvCfg<64>::WPB - 1) / WvCfg<64>::WPB);
-                hipLaunchKernelGGL(k_rc_wave<64>, dim3(wg), dim3(64 * WvCfg<64>::WPB), wlds64, st, t, k,
-                                   (const int *)(wlist + 2 * (size_t)wcap), (const int *)(wcount + 2), wcount + 5, wws, acc,
-                                   fixed_rank, maxrank, o, ctx->d_ovf);
+                hipLaunchKernelGGL(k_rc_wave<64>, dim3(wg), dim3(64 * WvCfg<64>::WPB), wlds64, ctx->side[1], t, k,
+                                   (const int *)(wlist + 2 * (size_t)wcap), (const int *)(wcount + 2), wcount + 6,
+                                   wws + wws_d / 2, acc, fixed_rank, maxrank, o, ctx->d_ovf);
+                wg = (int)std::min<long long>(wgrid, ntask);
+                hipLaunchKernelGGL(k_rc_wide, dim3(wg), dim3(64), sizeof(double) * WB_LDS, ctx->side[2], t, k,
+                                   (const int *)(wlist + 3 * (size_t)wcap), (const int *)(wcount + 3), wcount + 7, wbs,
+                                   acc, fixed_rank, maxrank, o, ctx->d_ovf);
+                // join
+                for (int i = 0; i < 3; i++) {
+                    HIPCHK(hipEventRecord(ctx->ev_join[i], ctx->side[i]));
+                    HIPCHK(hipStreamWaitEvent(st, ctx->ev_join[i], 0));
+                }
             }
             if (fast) {
                 int gf = (int)std::min<long long>(nsf, ntask);

@@ -337,8 +337,9 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
 }
 
 // Tasks of step k whose update runs on one wave: both factors low rank,
-// s = rc + ra in [1, WV_SMAX], rb <= WV_SMAX, NB <= 64 * WV_KMAX.  They are
-// appended to list 0 (s, rb <= 32), 1 (<= 48) or 2 (<= WV_SMAX); the order inside
+// s = rc + ra in [1, 128], rb <= WV_SMAX, NB <= 64 * WV_KMAX.  They are
+// appended to list 0 (s, rb <= 32), 1 (<= 48), 2 (<= WV_SMAX) or 3 (s <= 128,
+// rb <= WV_SMAX: k_rc_wide); the order inside
 // a list is irrelevant (tasks of one step are independent).  Listed tasks are
 // marked -1 in tasks_s; every other task keeps its k_classify value.
 __global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wcount, int cap)
@@ -353,9 +354,9 @@ __global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wco
     if (m - n < b || n - k < b || t.NB > 64 * WV_KMAX) return;
     const int ra = *rank_at(t, m, k), rc = *rank_at(t, m, n), rb = *rank_at(t, n, k);
     const int s = rc + ra;
-    if (ra > 0 && rb > 0 && s <= WV_SMAX && rb <= WV_SMAX) {
+    if (ra > 0 && rb > 0 && s <= 128 && rb <= WV_SMAX) {
         tasks_s[e] = -1;
-        const int l = (s <= 32 && rb <= 32) ? 0 : ((s <= 48 && rb <= 48) ? 1 : 2);
+        const int l = (s <= 32 && rb <= 32) ? 0 : ((s <= 48 && rb <= 48) ? 1 : (s <= WV_SMAX ? 2 : 3));
         wlist[(size_t)l * cap + atomicAdd(wcount + l, 1)] = (int)e;
     }
 }
@@ -635,6 +636,448 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                            Vl, SM, C);
             wv_sync();
             // V' from Out into columns nr .. 2nr-1 of the tile (flat copy)
+            {
+                const double *src = Out + (size_t)nr * NB;
+                double *d2 = C + (size_t)nr * NB;
+                const int tot = nr * NB;
+                int q = lane;
+                for (; q + 64 * 7 < tot; q += 64 * 8) {
+                    double v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) v[u] = src[q + 64 * u];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) d2[q + 64 * u] = v[u];
+                }
+                for (; q < tot; q += 64) d2[q] = src[q];
+            }
+        }
+        if (lane == 0) {
+            *rank_at(t, m, n) = nr;
+            unsigned long long mm = tdim(t, m), ss = s;
+            unsigned long long qa = opc('q', mm, ss, 0, 0);
+            unsigned long long qb = opc('m', ra, rb, mm, 0) + opc('m', ra, mm, rb, 0) + opc('q', mm, ss, 0, 0);
+            unsigned long long sv = opc('r', ss, ss, 2, 0) + opc('s', ss, 0, 0, 0) + ss * nr;
+            unsigned long long nu = opc('o', mm, nr, ss, 1), nv = opc('o', nr, mm, ss, 2);
+            oplog(o, NT, qa + qb + sv + nu + nv, m, n, false, false);
+            atomicAdd(o.stat + 4, qa + qb + sv + nu + nv);
+            atomicAdd(o.stat + 5, 1ULL);
+            unsigned long long words = 2ULL * NB * (ra + rc + nr) + 2ULL * NB * rb;
+            atomicAdd(o.stat + 6, 8ULL * words);
+        }
+        wv_sync();
+    }
+}
+
+// ===================== stacks wider than 64 (s <= 128) =====================
+
+#define WB_SM 128                       // largest stack width of the wide variant
+#define WB_LD (WB_SM + 1)               // LDS column stride of the Jacobi matrix
+#define WB_LDS (WB_SM * WB_LD + 2 * WB_SM)  // doubles of LDS (one wave per workgroup)
+
+__host__ __device__ inline size_t wb_slot_doubles(int NB)
+{
+    return (size_t)NB * 4 * WB_SM + 3 * (size_t)WB_SM * WB_SM;
+}
+
+// wv_geqrf_r for s <= 128: pivot row j lives on lane j & 63 of row block
+// j >> 6 (0 or 1); rows of block 0 are above every pivot of block 1.
+// R is written column-major (R(i, j) at Rout[i + j * WB_SM]).
+__device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout)
+{
+    const int lane = threadIdx.x & 63;
+    const int NB = A.NB;
+    for (int j0 = 0; j0 < s; j0 += WV_PW) {
+        const int jb = min(WV_PW, s - j0);
+        const bool first = (j0 == 0);
+        const int ps = j0 >> 6;  // row block of the panel's pivots (panels never straddle 64)
+        double a[WV_KMAX][WV_PW];
+#pragma unroll
+        for (int c = 0; c < WV_PW; c++) {
+            const double *col = A.col(j0 + c, first);
+#pragma unroll
+            for (int k = 0; k < WV_KMAX; k++) {
+                const int i = lane + 64 * k;
+                a[k][c] = (c < jb && i < NB) ? col[i] : 0.0;
+            }
+        }
+        double T[WV_PW][WV_PW];
+#pragma unroll
+        for (int r = 0; r < WV_PW; r++)
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++) T[r][c] = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < WV_PW; jj++) {
+            if (jj < jb) {
+                const int jl = (j0 + jj) & 63;
+                const double x0 = a[0][jj], x1 = a[1][jj];
+                // part of column jj strictly below the pivot, per row block
+                const double y0 = (ps == 0 && lane > jl) ? x0 : 0.0;
+                const double y1 = (ps == 0 || lane > jl) ? x1 : 0.0;
+                double p[WV_PW];
+#pragma unroll
+                for (int c = 0; c < WV_PW; c++)
+                    p[c] = y0 * (c == jj ? y0 : a[0][c]) + y1 * (c == jj ? y1 : a[1][c]);
+#pragma unroll
+                for (int k = 2; k < WV_KMAX; k++)
+#pragma unroll
+                    for (int c = 0; c < WV_PW; c++) p[c] = fma(a[k][jj], a[k][c], p[c]);
+                const double red = wrs4(p[0], p[1], p[2], p[3]);
+                double S[WV_PW], prow[WV_PW];
+#pragma unroll
+                for (int c = 0; c < WV_PW; c++) {
+                    S[c] = rdlane(red, 16 * c);
+                    prow[c] = rdlane(ps ? a[1][c] : a[0][c], jl);
+                }
+                const double alpha = prow[jj], xn2 = S[jj];
+                double tau = 0.0, beta = alpha, scal = 0.0;
+                if (xn2 > 0.0) {
+                    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+                    tau = (beta - alpha) / beta;
+                    scal = 1.0 / (alpha - beta);
+                }
+                const double v0 = (ps == 0 && lane == jl) ? 1.0 : y0 * scal;
+                const double v1 = (ps == 1 && lane == jl) ? 1.0 : y1 * scal;
+                a[0][jj] = v0;
+                a[1][jj] = v1;
+#pragma unroll
+                for (int k = 2; k < WV_KMAX; k++) a[k][jj] *= scal;
+#pragma unroll
+                for (int c = 0; c < WV_PW; c++) {
+                    if (c > jj) {
+                        const double td = tau * (prow[c] + S[c] * scal);
+#pragma unroll
+                        for (int k = 0; k < WV_KMAX; k++) a[k][c] = fma(-td, a[k][jj], a[k][c]);
+                    }
+                }
+                // R entries above the pivot and beta at the pivot; reflector below
+                if (ps == 0) {
+                    a[0][jj] = (lane == jl) ? beta : ((lane > jl) ? v0 : x0);
+                } else {
+                    a[0][jj] = x0;
+                    a[1][jj] = (lane == jl) ? beta : ((lane > jl) ? v1 : x1);
+                }
+#pragma unroll
+                for (int l = 0; l < WV_PW; l++) {
+                    if (l < jj) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int q = 0; q < WV_PW; q++)
+                            if (q >= l && q < jj) acc = fma(T[l][q], prow[q] + S[q] * scal, acc);
+                        T[l][jj] = -tau * acc;
+                    }
+                }
+                T[jj][jj] = tau;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < WV_PW; c++)
+            if (c < jb) {
+#pragma unroll
+                for (int blk = 0; blk < 2; blk++) {
+                    const int i = lane + 64 * blk;
+                    if (i < s) Rout[i + (size_t)(j0 + c) * WB_SM] = (i <= j0 + c) ? a[blk][c] : 0.0;
+                }
+            }
+        // reflector entries of row blocks 0 and 1 (1 at the pivot, 0 above)
+        double m0[WV_PW], m1[WV_PW];
+#pragma unroll
+        for (int c = 0; c < WV_PW; c++) {
+            const int gl = (j0 + c) & 63;
+            const bool in = c < jb;
+            m0[c] = (!in || ps == 1) ? 0.0 : ((lane == gl) ? 1.0 : (lane > gl ? a[0][c] : 0.0));
+            m1[c] = !in ? 0.0 : (ps == 0 ? a[1][c] : ((lane == gl) ? 1.0 : (lane > gl ? a[1][c] : 0.0)));
+        }
+        for (int t = j0 + jb; t < s; t++) {
+            const double *col = A.col(t, first);
+            double b[WV_KMAX];
+#pragma unroll
+            for (int k = 0; k < WV_KMAX; k++) {
+                const int i = lane + 64 * k;
+                b[k] = (i < NB) ? col[i] : 0.0;
+            }
+            double p[WV_PW];
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++) {
+                double acc = fma(m0[c], b[0], m1[c] * b[1]);
+#pragma unroll
+                for (int k = 2; k < WV_KMAX; k++) acc = fma(a[k][c], b[k], acc);
+                p[c] = acc;
+            }
+            const double red = wrs4(p[0], p[1], p[2], p[3]);
+            double d[WV_PW], w[WV_PW];
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++) d[c] = rdlane(red, 16 * c);
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int q = 0; q <= c; q++) acc = fma(T[q][c], d[q], acc);
+                w[c] = acc;
+            }
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++) {
+                b[0] = fma(-m0[c], w[c], b[0]);
+                b[1] = fma(-m1[c], w[c], b[1]);
+            }
+#pragma unroll
+            for (int k = 2; k < WV_KMAX; k++) {
+                double acc = b[k];
+#pragma unroll
+                for (int c = 0; c < WV_PW; c++) acc = fma(-a[k][c], w[c], acc);
+                b[k] = acc;
+            }
+            double *out = A.W + (size_t)t * NB;
+#pragma unroll
+            for (int k = 0; k < WV_KMAX; k++) {
+                const int i = lane + 64 * k;
+                if (i < NB) out[i] = b[k];
+            }
+        }
+    }
+}
+
+// Tasks of step k with 64 < s <= 128 and rb <= 64 (list 3): one wave each.
+__global__ void __launch_bounds__(64, 1)
+k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double *ws, double acc, int fixed_rk,
+          int maxrank, OpLog o, int *ovf)
+{
+    extern __shared__ double smem[];
+    const int lane = threadIdx.x & 63;
+    double *L = smem;
+    double *sg = L + WB_SM * WB_LD;
+    const int NB = t.NB, NT = t.NT;
+    double *Wq = ws + (size_t)blockIdx.x * wb_slot_doubles(NB);
+    double *Qn = Wq + (size_t)NB * WB_SM;
+    double *Out = Qn + (size_t)NB * WB_SM;
+    double *Rp = Out + (size_t)NB * 2 * WB_SM;
+    double *Rq = Rp + WB_SM * WB_SM;
+    double *Ku = Rq + WB_SM * WB_SM;
+    const int cnt = uni(*wcount);
+    for (;;) {
+        int idx = 0;
+        if (lane == 0) idx = atomicAdd(wnext, 1);
+        idx = uni(idx);
+        if (idx >= cnt) break;
+        const int e = uni(wlist[idx]);
+        int hi, lo;
+        tri_decode(e, hi, lo);
+        const int m = uni(k + 1 + hi), n = uni(k + 1 + lo);
+        const int ra = uni(*rank_at(t, m, k)), rc = uni(*rank_at(t, m, n)), rb = uni(*rank_at(t, n, k));
+        const int s = rc + ra;
+        const double *Au = lr_tile(t, m, k), *Av = Au + (size_t)NB * ra;
+        const double *Bu = lr_tile(t, n, k), *Bv = Bu + (size_t)NB * rb;
+        double *C = lr_tile(t, m, n);
+        const int i16 = lane & 15, k4 = lane >> 4;
+
+        // ---- X = Av^T Bv (ra <= 128, rb <= 64): one 16-row band of X at a time ----
+        {
+            double *Xl = L;  // Xl[b * WB_SM + a]
+            for (int q = lane; q < 64 * WB_SM; q += 64) Xl[q] = 0.0;
+            wv_sync();
+            for (int ta = 0; 16 * ta < ra; ta++) {
+                wv_d4 xacc[4];
+#pragma unroll
+                for (int tb = 0; tb < 4; tb++) xacc[tb] = wv_d4{0.0, 0.0, 0.0, 0.0};
+                const int a = 16 * ta + i16;
+                for (int r0 = 0; r0 < NB; r0 += 16) {
+                    double av[4], bv[4][4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int row = r0 + 4 * u + k4;
+                        av[u] = (a < ra && row < NB) ? Av[row + (size_t)a * NB] : 0.0;
+#pragma unroll
+                        for (int tb = 0; tb < 4; tb++) {
+                            const int bb = 16 * tb + i16;
+                            bv[u][tb] = (16 * tb < rb && bb < rb && row < NB) ? Bv[row + (size_t)bb * NB] : 0.0;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+#pragma unroll
+                        for (int tb = 0; tb < 4; tb++)
+                            if (16 * tb < rb) xacc[tb] = mfma16(av[u], bv[u][tb], xacc[tb]);
+                }
+#pragma unroll
+                for (int tb = 0; tb < 4; tb++)
+                    if (16 * tb < rb) {
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int aa = 16 * ta + k4 + 4 * r, bb = 16 * tb + i16;
+                            if (aa < ra && bb < rb) Xl[bb * WB_SM + aa] = xacc[tb][r];
+                        }
+                    }
+            }
+            wv_sync();
+        }
+        // ---- Qn = -Bu X^T ----
+        wv_tall_mm<WB_SM>(NB, rb, ra, -1.0, [&](int c) { return Bu + (size_t)c * NB; }, L, WB_SM, Qn);
+        wv_sync();
+        // ---- R factors ----
+        for (int side = 0; side < 2; side++) {
+            WvCols S{side ? C + (size_t)NB * rc : C, side ? (const double *)Qn : Au, rc, Wq, NB};
+            wb_geqrf_r(s, S, side ? Rq : Rp);
+        }
+        wv_sync();
+        // ---- M^T = Rq Rp^T on the matrix cores (K = Rp^T = Rp's column-major
+        //      storage read row-wise, staged in LDS) into W (ld s); column c of
+        //      M^T (= row c of M) then goes to the Jacobi matrix ----
+        double *J = L;
+        {
+            for (int q = lane; q < s * WB_SM; q += 64) {
+                const int c = q % WB_SM;
+                L[q] = (c < s) ? Rp[q] : 0.0;  // K[l][c] = Rp(c, l)
+            }
+            wv_sync();
+            wv_tall_mm<WB_SM>(s, s, s, 1.0, [&](int l) { return (const double *)Rq + (size_t)l * WB_SM; }, L, WB_SM,
+                              Wq);
+            wv_sync();
+            for (int q = lane; q < s * WB_LD; q += 64) {
+                const int c = q / WB_LD, r = q - c * WB_LD;
+                J[q] = (r < s) ? Wq[r + (size_t)c * s] : 0.0;
+            }
+            wv_sync();
+        }
+        // ---- one-sided Jacobi, columns in LDS, one column pair per lane ----
+        {
+            const int npad = s + (s & 1), npairs = npad >> 1;
+            const double tol = 4.0e-16 * sqrt((double)s);
+            double f2 = 0.0;
+            for (int c = lane; c < s; c += 64) {
+                const double *x = J + c * WB_LD;
+                for (int q = 0; q < s; q++) f2 = fma(x[q], x[q], f2);
+            }
+            const double nz = (2.3e-16 * s) * (2.3e-16 * s) * wsum(f2);
+            for (int sweep = 0; sweep < 60; sweep++) {
+                int rot_any = 0;
+                for (int r = 0; r < npad - 1; r++) {
+                    if (lane < npairs) {
+                        const int pa = lane, pb = npad - 1 - lane;
+                        const int ca = (pa == 0) ? 0 : ((pa - 1 + r) % (npad - 1)) + 1;
+                        const int cb = (pb == 0) ? 0 : ((pb - 1 + r) % (npad - 1)) + 1;
+                        if (ca < s && cb < s) {
+                            double *x = J + ca * WB_LD, *y = J + cb * WB_LD;
+                            double al0 = 0.0, al1 = 0.0, be0 = 0.0, be1 = 0.0, ga0 = 0.0, ga1 = 0.0;
+                            int q = 0;
+                            for (; q + 1 < s; q += 2) {
+                                const double x0 = x[q], x1 = x[q + 1], y0 = y[q], y1 = y[q + 1];
+                                al0 = fma(x0, x0, al0);
+                                al1 = fma(x1, x1, al1);
+                                be0 = fma(y0, y0, be0);
+                                be1 = fma(y1, y1, be1);
+                                ga0 = fma(x0, y0, ga0);
+                                ga1 = fma(x1, y1, ga1);
+                            }
+                            if (q < s) {
+                                const double x0 = x[q], y0 = y[q];
+                                al0 = fma(x0, x0, al0);
+                                be0 = fma(y0, y0, be0);
+                                ga0 = fma(x0, y0, ga0);
+                            }
+                            const double al = al0 + al1, be = be0 + be1, ga = ga0 + ga1;
+                            double cs, sn;
+                            if (fmin(al, be) > nz && jacobi_rot(al, be, ga, tol, cs, sn)) {
+                                rot_any = 1;
+                                for (int q2 = 0; q2 < s; q2++) {
+                                    const double xv = x[q2], yv = y[q2];
+                                    x[q2] = cs * xv - sn * yv;
+                                    y[q2] = sn * xv + cs * yv;
+                                }
+                            }
+                        }
+                    }
+                    wv_sync();
+                }
+                if (__ballot(rot_any) == 0) break;
+            }
+        }
+        // ---- singular values (columns lane and lane + 64), order, rank ----
+        for (int c = lane; c < s; c += 64) {
+            const double *x = J + c * WB_LD;
+            double v = 0.0;
+            for (int q = 0; q < s; q++) v = fma(x[q], x[q], v);
+            sg[c] = sqrt(v);
+        }
+        wv_sync();
+        int nr = 0;
+        {
+            int cntv = 0;
+            for (int c = lane; c < s; c += 64) cntv += (sg[c] >= acc);
+            nr = (fixed_rk > 0) ? (fixed_rk < s ? fixed_rk : s) : (int)wsum((double)cntv);
+        }
+        if (nr > maxrank) {
+            if (lane == 0) atomicAdd(ovf, 1);
+            nr = maxrank;
+        }
+        nr = uni(nr);
+        if (nr > 0) {
+            // selected columns, normalized, to Out (column pos, ld WB_SM), then
+            // row-major into LDS: Vl[l * WB_SM + pos]
+            for (int c = lane; c < s; c += 64) {
+                const double sc = sg[c];
+                int pos = 0;
+                for (int q = 0; q < s; q++) {
+                    const double o2 = sg[q];
+                    pos += (o2 > sc) || (o2 == sc && q < c);
+                }
+                if (pos < nr) {
+                    const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
+                    const double *x = J + c * WB_LD;
+                    for (int q = 0; q < s; q++) Out[q + (size_t)pos * WB_SM] = x[q] * inv;
+                }
+            }
+            wv_sync();
+            double *Vl = L;
+            for (int q = lane; q < s * WB_SM; q += 64) {
+                const int l = q / WB_SM, j = q - l * WB_SM;
+                Vl[q] = (j < nr) ? Out[l + (size_t)j * WB_SM] : 0.0;
+            }
+            wv_sync();
+            // Ku = Rq^T Vs_r: lane c computes rows c, c + 64; 32 columns at a time
+            for (int c = lane; c < s; c += 64) {
+                for (int j0 = 0; j0 < nr; j0 += 32) {
+                    double ac[32];
+#pragma unroll
+                    for (int j = 0; j < 32; j++) ac[j] = 0.0;
+                    for (int l = 0; l <= c; l++) {
+                        const double rq = Rq[l + (size_t)c * WB_SM];
+                        const double *vr = Vl + l * WB_SM + j0;
+#pragma unroll
+                        for (int j = 0; j < 32; j++) ac[j] = fma(rq, vr[j], ac[j]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 32; j++)
+                        if (j0 + j < nr) Ku[c * WB_SM + j0 + j] = ac[j];
+                }
+            }
+            wv_sync();
+            // Kv = Rq^-1 Vs_r in place; lane solves columns lane, lane + 64
+            double dmax = 0.0;
+            for (int q = 0; q < s; q++) dmax = fmax(dmax, fabs(Rq[q + (size_t)q * WB_SM]));
+            const double thr = 1e-14 * dmax;
+            for (int j = lane; j < nr; j += 64) {
+                for (int q = s - 1; q >= 0; q--) {
+                    double v = Vl[q * WB_SM + j];
+                    for (int l = q + 1; l < s; l++) v = fma(-Rq[q + (size_t)l * WB_SM], Vl[l * WB_SM + j], v);
+                    const double d = Rq[q + (size_t)q * WB_SM];
+                    Vl[q * WB_SM + j] = (fabs(d) > thr) ? v / d : 0.0;
+                }
+            }
+            wv_sync();
+            // ---- outputs as in k_rc_wave ----
+            wv_tall_mm<WB_SM>(NB, s, nr, 1.0,
+                              [&](int c) { return c < rc ? C + (size_t)(rc + c) * NB : Qn + (size_t)(c - rc) * NB; },
+                              Vl, WB_SM, Out + (size_t)nr * NB);
+            wv_sync();
+            for (int q = lane; q < s * WB_SM; q += 64) {
+                const int c = q / WB_SM, j = q - c * WB_SM;
+                Vl[q] = (j < nr) ? Ku[c * WB_SM + j] : 0.0;
+            }
+            wv_sync();
+            wv_tall_mm<WB_SM>(
+                NB, s, nr, 1.0,
+                [&](int c) { return c < rc ? (const double *)C + (size_t)c * NB : Au + (size_t)(c - rc) * NB; }, Vl,
+                WB_SM, C);
+            wv_sync();
             {
                 const double *src = Out + (size_t)nr * NB;
                 double *d2 = C + (size_t)nr * NB;

@@ -298,6 +298,74 @@ __global__ void __launch_bounds__(BS) k_potrf_diag2(TLR t, int k, int j0, int jb
     }
 }
 
+// Same contract as k_potrf_diag2, one wave: left-looking Cholesky with the
+// block row-major in LDS (lane i owns row i; the pivot row is read as an LDS
+// broadcast and the pivot itself by readlane), then W = L^{-1} with lane c
+// owning column c (forward substitution, L rows read as broadcasts).
+#define PLR (PBK + 2)
+__global__ void __launch_bounds__(64) k_potrf_diag3(TLR t, int k, int j0, int jb, double *Winv, int *info, OpLog o)
+{
+    if (j0 == 0 && threadIdx.x == 0) oplog(o, t.NT, opc('c', tdim(t, k), 0, 0, 0), k, k, true, true);
+    __shared__ double Lr[PBK * PLR];  // Lr[i * PLR + c] = L(i, c)
+    __shared__ double Wc[PBK * PLR];  // Wc[c * PLR + i] = W(i, c)
+    const int NB = t.NB, lane = threadIdx.x;
+    double *A = band_tile(t, k, k) + j0 + (size_t)j0 * NB;
+    for (int c = 0; c < jb; c++) Lr[lane * PLR + c] = (lane < jb && lane >= c) ? A[lane + (size_t)c * NB] : 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const double *my = Lr + lane * PLR;
+    for (int j = 0; j < jb; j++) {
+        // s_i = A(i, j) - sum_{l<j} L(i, l) L(j, l), i >= j
+        const double *rj = Lr + j * PLR;
+        double s0 = 0.0, s1 = 0.0;
+        int l = 0;
+        for (; l + 1 < j; l += 2) {
+            s0 = fma(my[l], rj[l], s0);
+            s1 = fma(my[l + 1], rj[l + 1], s1);
+        }
+        if (l < j) s0 = fma(my[l], rj[l], s0);
+        const double v = my[j] - (s0 + s1);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)lo32(v), j);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)hi32(v), j);
+        const double d2 = mk64(lo, hi);
+        if (!(d2 > 0.0)) {
+            if (lane == 0) atomicCAS(info, 0, k * NB + j0 + j + 1);
+            return;
+        }
+        const double d = sqrt(d2);
+        if (lane == j) Lr[lane * PLR + j] = d;
+        else if (lane > j && lane < jb) Lr[lane * PLR + j] = v / d;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // W(i, c) = (delta_ic - sum_{l=c}^{i-1} L(i, l) W(l, c)) / L(i, i), lane c
+    {
+        const int c = lane;
+        double *wc = Wc + c * PLR;
+        for (int i = 0; i < jb; i++) {
+            const double *ri = Lr + i * PLR;
+            double s0 = 0.0, s1 = 0.0;
+            int l = c;
+            for (; l + 1 < i; l += 2) {
+                s0 = fma(ri[l], wc[l], s0);
+                s1 = fma(ri[l + 1], wc[l + 1], s1);
+            }
+            if (l < i) s0 = fma(ri[l], wc[l], s0);
+            const double lii = ri[i];
+            double w = 0.0;
+            if (c < jb && i >= c) w = (i == c) ? 1.0 / lii : -(s0 + s1) / lii;
+            wc[i] = w;
+        }
+    }
+    double *Wo = Winv + (size_t)(j0 / PBK) * PBK * PBK;
+    for (int c = 0; c < jb; c++) {
+        if (lane < jb && lane >= c) A[lane + (size_t)c * NB] = Lr[lane * PLR + c];
+        if (lane < jb) Wo[lane + c * PBK] = Wc[c * PLR + lane];
+    }
+}
+
 // panel rows below the diagonal block: X = C W^T (W = L^{-1}), in place;
 // grid = ceil(rows / PBK)
 __global__ void __launch_bounds__(BS) k_potrf_panel2(TLR t, int k, int j0, int jb, const double *Winv)
@@ -381,6 +449,80 @@ __global__ void __launch_bounds__(BS) k_trsm_lr2(TLR t, int k, const double *Win
 #pragma unroll
                 for (int q = 0; q < 4; q++) x[q] = fma(w, Vb[l + (cg * 4 + q) * PBK], x[q]);
             }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int c = cg * 4 + q;
+            if (i < nb && c < nc) V[b0 + i + (size_t)c * NB] = x[q];
+        }
+        __syncthreads();
+    }
+}
+
+// low-rank tiles m = k+band .. NT-1: V <- L^{-1} V, 16 columns of one tile
+// per workgroup.  Row block b (64 rows) is X_b = W_b (V_b - L_b,0:b X_0:b): the
+// sum runs over 64-deep chunks of L and of the solved rows staged in LDS (so
+// every global load of a chunk is in flight at once), then the inverted
+// diagonal block W_b is applied from LDS.  grid (ceil(cap/16), count).
+#define TL3 (PBK + 1)
+__global__ void __launch_bounds__(BS) k_trsm_lr3(TLR t, int k, const double *Winv, OpLog o)
+{
+    __shared__ double Ls[PBK * TL3];   // Ls[kk * TL3 + i] = L(b0 + i, l0 + kk); also W_b
+    __shared__ double Xs[PBK * 17];    // Xs[kk * 17 + c]  = X(l0 + kk, c); also V_b
+    const int NB = t.NB, tid = threadIdx.x;
+    const int m = k + t.band_size + blockIdx.y;
+    const int r = *rank_at(t, m, k);
+    if (blockIdx.x == 0 && tid == 0) oplog(o, t.NT, opc('t', tdim(t, m), r, 1, 0), m, k, false, m - k == 1);
+    const int c0 = blockIdx.x * 16;
+    if (c0 >= r) return;
+    const int nc = (r - c0) < 16 ? (r - c0) : 16;
+    double *V = lr_tile(t, m, k) + (size_t)NB * r + (size_t)c0 * NB;
+    const double *L = band_tile(t, k, k);
+    const int i = tid % PBK, cg = tid / PBK;  // row i, columns 4 cg .. 4 cg + 3
+    for (int b0 = 0; b0 < NB; b0 += PBK) {
+        const int nb = (NB - b0) < PBK ? (NB - b0) : PBK;
+        double acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int c = cg * 4 + q;
+            acc[q] = (i < nb && c < nc) ? V[b0 + i + (size_t)c * NB] : 0.0;
+        }
+        for (int l0 = 0; l0 < b0; l0 += PBK) {
+            // stage L(b0:b0+64, l0:l0+64) and X(l0:l0+64, 0:16)
+#pragma unroll 4
+            for (int e = tid; e < PBK * PBK; e += BS) {
+                const int ii = e % PBK, kk = e / PBK;
+                Ls[kk * TL3 + ii] = (ii < nb) ? L[b0 + ii + (size_t)(l0 + kk) * NB] : 0.0;
+            }
+#pragma unroll
+            for (int e = tid; e < PBK * 16; e += BS) {
+                const int kk = e % PBK, c = e / PBK;
+                Xs[kk * 17 + c] = (c < nc) ? V[l0 + kk + (size_t)c * NB] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll 8
+            for (int kk = 0; kk < PBK; kk++) {
+                const double lv = Ls[kk * TL3 + i];
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[q] = fma(-lv, Xs[kk * 17 + cg * 4 + q], acc[q]);
+            }
+            __syncthreads();
+        }
+        // X_b = W_b acc: W_b (lower triangular 64 x 64) into Ls, acc into Xs
+        const double *Wb = Winv + (size_t)(b0 / PBK) * PBK * PBK;
+#pragma unroll 4
+        for (int e = tid; e < PBK * PBK; e += BS) {
+            const int ii = e % PBK, l = e / PBK;
+            Ls[l * TL3 + ii] = Wb[ii + (size_t)l * PBK];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) Xs[i * 17 + cg * 4 + q] = acc[q];
+        __syncthreads();
+        double x[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int l = 0; l <= i && l < nb; l++) {
+            const double w = Ls[l * TL3 + i];
+#pragma unroll
+            for (int q = 0; q < 4; q++) x[q] = fma(w, Xs[l * 17 + cg * 4 + q], x[q]);
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
