@@ -39,7 +39,8 @@ struct hicma_ctx {
     std::vector<hipEvent_t> prof_ev; // pairs
     // the recompression variants of one panel step run concurrently on side
     // streams (default priority), forked from and joined back to `stream`
-    hipStream_t side[3];
+    // (the device exposes 4 hardware queues per process: 4 streams in all)
+    hipStream_t side[3];             // [2]: syrk(k) + potrf(k+1), one step ahead
     hipEvent_t ev_fork, ev_join[3];
 };
 
@@ -491,10 +492,9 @@ extern "C" int parsec_band_regenerate(hicma_ctx *ctx, int uplo, hicma_tlr_desc *
 // blocked Cholesky of diagonal tile k: per 64-column block, factor + invert
 // the diagonal block in LDS, panel = C W^T, lower trailing update.  The block
 // inverses (Winv, NB/64 blocks of 64x64) are reused by the low-rank trsm.
-static int launch_potrf(hicma_tlr_desc *A, int k, double *Winv, OpLog o)
+static int launch_potrf(hicma_tlr_desc *A, int k, double *Winv, OpLog o, hipStream_t s)
 {
     TLR &t = A->t;
-    hipStream_t s = A->ctx->stream;
     for (int j0 = 0; j0 < t.NB; j0 += PBK) {
         int jb = std::min(PBK, t.NB - j0);
         hipLaunchKernelGGL(k_potrf_diag3, dim3(1), dim3(64), 0, s, t, k, j0, jb, Winv, A->ctx->d_info, o);
@@ -568,6 +568,9 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     // the s <= 48 variant (4 waves each) and of the s <= 64 variant (2 waves)
     const bool wave = (NB <= 64 * WV_KMAX);
     const int wgrid = wave ? ctx->ncu : 0;  // times WvCfg<SM>::BPC resident workgroups
+    // CUs left to the lookahead stream (syrk(k) + potrf(k+1)) while the
+    // recompression kernels of step k run
+    const int wres = (wave && ctx->ncu > 64) ? 16 : 0;
     const size_t wstride = wv_slot_doubles(NB);
     const int wcap = (int)(maxtask > 0 ? maxtask : 1);
     // four concurrent variants (32, 48, 64 on side streams), each with up to
@@ -619,9 +622,13 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     };
     const int ntg = (NB + GT - 1) / GT;
     const int ntl = ntg * (ntg + 1) / 2;
+    // Lookahead: syrk(k) and potrf(k+1) touch only diagonal tiles and the gemm
+    // updates of step k only off-diagonal ones, so the former run on a side
+    // stream while the latter run, and are joined back before trsm(k+1).
+    hipStream_t sd = ctx->side[2];
+    if (NT > 0 && (rc = launch_potrf(A, 0, Winv, o, st))) return rc;
     for (int k = 0; k < NT; k++) {
         mark();
-        if ((rc = launch_potrf(A, k, Winv, o))) return rc;
         mark();
         int nbnd = std::min(b - 1, NT - 1 - k);
         int nlr = NT - k - b;
@@ -630,11 +637,14 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
         if (nlr > 0)
             hipLaunchKernelGGL(k_trsm_lr3, dim3((cap + 15) / 16, nlr), dim3(BS), 0, st, t, k, (const double *)Winv, o);
         mark();
-        if (nbnd > 0) hipLaunchKernelGGL(k_syrk_band, dim3(ntl, nbnd), dim3(BS), 0, st, t, k, o);
+        HIPCHK(hipEventRecord(ctx->ev_join[2], st));
+        HIPCHK(hipStreamWaitEvent(sd, ctx->ev_join[2], 0));
+        if (nbnd > 0) hipLaunchKernelGGL(k_syrk_band, dim3(ntl, nbnd), dim3(BS), 0, sd, t, k, o);
         if (nlr > 0) {
-            hipLaunchKernelGGL(k_syrk_lr_prep, dim3(nlr), dim3(BS), 0, st, t, k, Yws, Zws, o);
-            hipLaunchKernelGGL(k_syrk_lr_update, dim3(ntl, nlr), dim3(BS), 0, st, t, k, (const double *)Yws);
+            hipLaunchKernelGGL(k_syrk_lr_prep, dim3(nlr), dim3(BS), 0, sd, t, k, Yws, Zws, o);
+            hipLaunchKernelGGL(k_syrk_lr_update, dim3(ntl, nlr), dim3(BS), 0, sd, t, k, (const double *)Yws);
         }
+        if (k + 1 < NT && (rc = launch_potrf(A, k + 1, Winv, o, sd))) return rc;
         mark();
         if (nbnd >= 2)
             hipLaunchKernelGGL(k_gemm_dense, dim3(ntg * ntg, nbnd * (nbnd - 1) / 2), dim3(BS), 0, st, t, k, o);
@@ -659,27 +669,28 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
                 HIPCHK(hipMemsetAsync(wcount, 0, 8 * sizeof(int), st));
                 hipLaunchKernelGGL(k_classify_wave, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k,
                                    tasks_s, wlist, wcount, wcap);
-                // fork: variants 48, 64 and the wide one on the side streams
+                // fork: variants 32 and 48 on the context stream, 64 and the
+                // wide one on side streams 0 and 1
                 HIPCHK(hipEventRecord(ctx->ev_fork, st));
-                for (int i = 0; i < 3; i++) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->ev_fork, 0));
-                int wg = (int)std::min<long long>(wgrid * WvCfg<32>::BPC, (ntask + WvCfg<32>::WPB - 1) / WvCfg<32>::WPB);
+                for (int i = 0; i < 2; i++) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->ev_fork, 0));
+                int wg = (int)std::min<long long>((wgrid - wres) * WvCfg<32>::BPC, (ntask + WvCfg<32>::WPB - 1) / WvCfg<32>::WPB);
                 hipLaunchKernelGGL(k_rc_wave<32>, dim3(wg), dim3(64 * WvCfg<32>::WPB), wlds32, st, t, k,
                                    (const int *)wlist, (const int *)wcount, wcount + 4, wws, acc, fixed_rank, maxrank,
                                    o, ctx->d_ovf);
-                wg = (int)std::min<long long>(wgrid * WvCfg<48>::BPC, (ntask + WvCfg<48>::WPB - 1) / WvCfg<48>::WPB);
-                hipLaunchKernelGGL(k_rc_wave<48>, dim3(wg), dim3(64 * WvCfg<48>::WPB), wlds48, ctx->side[0], t, k,
+                wg = (int)std::min<long long>((wgrid - wres) * WvCfg<48>::BPC, (ntask + WvCfg<48>::WPB - 1) / WvCfg<48>::WPB);
+                hipLaunchKernelGGL(k_rc_wave<48>, dim3(wg), dim3(64 * WvCfg<48>::WPB), wlds48, st, t, k,
                                    (const int *)(wlist + wcap), (const int *)(wcount + 1), wcount + 5, wws + wws_d / 4,
                                    acc, fixed_rank, maxrank, o, ctx->d_ovf);
-                wg = (int)std::min<long long>(wgrid * WvCfg<64>::BPC, (ntask + WvCfg<64>::WPB - 1) / WvCfg<64>::WPB);
-                hipLaunchKernelGGL(k_rc_wave<64>, dim3(wg), dim3(64 * WvCfg<64>::WPB), wlds64, ctx->side[1], t, k,
+                wg = (int)std::min<long long>((wgrid - wres) * WvCfg<64>::BPC, (ntask + WvCfg<64>::WPB - 1) / WvCfg<64>::WPB);
+                hipLaunchKernelGGL(k_rc_wave<64>, dim3(wg), dim3(64 * WvCfg<64>::WPB), wlds64, ctx->side[0], t, k,
                                    (const int *)(wlist + 2 * (size_t)wcap), (const int *)(wcount + 2), wcount + 6,
                                    wws + wws_d / 2, acc, fixed_rank, maxrank, o, ctx->d_ovf);
-                wg = (int)std::min<long long>(wgrid, ntask);
-                hipLaunchKernelGGL(k_rc_wide, dim3(wg), dim3(64), sizeof(double) * WB_LDS, ctx->side[2], t, k,
+                wg = (int)std::min<long long>(wgrid - wres, ntask);
+                hipLaunchKernelGGL(k_rc_wide, dim3(wg), dim3(64), sizeof(double) * WB_LDS, ctx->side[1], t, k,
                                    (const int *)(wlist + 3 * (size_t)wcap), (const int *)(wcount + 3), wcount + 7, wbs,
                                    acc, fixed_rank, maxrank, o, ctx->d_ovf);
                 // join
-                for (int i = 0; i < 3; i++) {
+                for (int i = 0; i < 2; i++) {
                     HIPCHK(hipEventRecord(ctx->ev_join[i], ctx->side[i]));
                     HIPCHK(hipStreamWaitEvent(st, ctx->ev_join[i], 0));
                 }
@@ -697,6 +708,8 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
                 ctx->prof_ev.push_back(e1);
             }
         }
+        HIPCHK(hipEventRecord(ctx->ev_join[2], sd));
+        HIPCHK(hipStreamWaitEvent(st, ctx->ev_join[2], 0));
         HIPCHK(hipGetLastError());
     }
     mark();

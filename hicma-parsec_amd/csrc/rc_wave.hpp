@@ -259,15 +259,38 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
         for (int c = 0; c < WV_PW; c++)
             v0[c] = (c < jb) ? ((lane == j0 + c) ? 1.0 : (lane > j0 + c ? a[0][c] : 0.0)) : 0.0;
         // ---- trailing columns, two at a time: b -= V (T^T (V^T b)) ----
-        for (int t = j0 + jb; t < s; t += 2) {
+        // the next pair is loaded before the current one is processed, so
+        // its memory latency overlaps the reduction and update
+        double n0[WV_KMAX], n1[WV_KMAX];
+        {
+            const int t = j0 + jb;
             const bool two = (t + 1 < s);
-            const double *col0 = A.col(t, first), *col1 = A.col(two ? t + 1 : t, first);
-            double b0[WV_KMAX], b1[WV_KMAX];
+            const double *col0 = A.col(t < s ? t : j0, first), *col1 = A.col(two ? t + 1 : j0, first);
 #pragma unroll
             for (int k = 0; k < WV_KMAX; k++) {
                 const int i = lane + 64 * k;
-                b0[k] = (i < NB) ? col0[i] : 0.0;
-                b1[k] = (two && i < NB) ? col1[i] : 0.0;
+                n0[k] = (t < s && i < NB) ? col0[i] : 0.0;
+                n1[k] = (two && i < NB) ? col1[i] : 0.0;
+            }
+        }
+        for (int t = j0 + jb; t < s; t += 2) {
+            const bool two = (t + 1 < s);
+            double b0[WV_KMAX], b1[WV_KMAX];
+#pragma unroll
+            for (int k = 0; k < WV_KMAX; k++) {
+                b0[k] = n0[k];
+                b1[k] = n1[k];
+            }
+            {
+                const int tn = t + 2;
+                const bool nx0 = tn < s, nx1 = tn + 1 < s;
+                const double *col0 = A.col(nx0 ? tn : t, first), *col1 = A.col(nx1 ? tn + 1 : t, first);
+#pragma unroll
+                for (int k = 0; k < WV_KMAX; k++) {
+                    const int i = lane + 64 * k;
+                    n0[k] = (nx0 && i < NB) ? col0[i] : 0.0;
+                    n1[k] = (nx1 && i < NB) ? col1[i] : 0.0;
+                }
             }
             double p[8];
 #pragma unroll
@@ -787,13 +810,28 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout)
             m0[c] = (!in || ps == 1) ? 0.0 : ((lane == gl) ? 1.0 : (lane > gl ? a[0][c] : 0.0));
             m1[c] = !in ? 0.0 : (ps == 0 ? a[1][c] : ((lane == gl) ? 1.0 : (lane > gl ? a[1][c] : 0.0)));
         }
-        for (int t = j0 + jb; t < s; t++) {
-            const double *col = A.col(t, first);
-            double b[WV_KMAX];
+        double nb_[WV_KMAX];
+        {
+            const int t = j0 + jb;
+            const double *col = A.col(t < s ? t : j0, first);
 #pragma unroll
             for (int k = 0; k < WV_KMAX; k++) {
                 const int i = lane + 64 * k;
-                b[k] = (i < NB) ? col[i] : 0.0;
+                nb_[k] = (t < s && i < NB) ? col[i] : 0.0;
+            }
+        }
+        for (int t = j0 + jb; t < s; t++) {
+            double b[WV_KMAX];
+#pragma unroll
+            for (int k = 0; k < WV_KMAX; k++) b[k] = nb_[k];
+            {
+                const bool nx = t + 1 < s;
+                const double *col = A.col(nx ? t + 1 : t, first);
+#pragma unroll
+                for (int k = 0; k < WV_KMAX; k++) {
+                    const int i = lane + 64 * k;
+                    nb_[k] = (nx && i < NB) ? col[i] : 0.0;
+                }
             }
             double p[WV_PW];
 #pragma unroll

@@ -310,7 +310,14 @@ __global__ void __launch_bounds__(64) k_potrf_diag3(TLR t, int k, int j0, int jb
     __shared__ double Wc[PBK * PLR];  // Wc[c * PLR + i] = W(i, c)
     const int NB = t.NB, lane = threadIdx.x;
     double *A = band_tile(t, k, k) + j0 + (size_t)j0 * NB;
-    for (int c = 0; c < jb; c++) Lr[lane * PLR + c] = (lane < jb && lane >= c) ? A[lane + (size_t)c * NB] : 0.0;
+    {
+        // all 64 loads in flight before the LDS stores
+        double row[PBK];
+#pragma unroll
+        for (int c = 0; c < PBK; c++) row[c] = (c < jb && lane < jb && lane >= c) ? A[lane + (size_t)c * NB] : 0.0;
+#pragma unroll
+        for (int c = 0; c < PBK; c++) Lr[lane * PLR + c] = row[c];
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -318,14 +325,14 @@ __global__ void __launch_bounds__(64) k_potrf_diag3(TLR t, int k, int j0, int jb
     for (int j = 0; j < jb; j++) {
         // s_i = A(i, j) - sum_{l<j} L(i, l) L(j, l), i >= j
         const double *rj = Lr + j * PLR;
-        double s0 = 0.0, s1 = 0.0;
+        double s4[4] = {0.0, 0.0, 0.0, 0.0};
         int l = 0;
-        for (; l + 1 < j; l += 2) {
-            s0 = fma(my[l], rj[l], s0);
-            s1 = fma(my[l + 1], rj[l + 1], s1);
+        for (; l + 7 < j; l += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; u++) s4[u & 3] = fma(my[l + u], rj[l + u], s4[u & 3]);
         }
-        if (l < j) s0 = fma(my[l], rj[l], s0);
-        const double v = my[j] - (s0 + s1);
+        for (; l < j; l++) s4[0] = fma(my[l], rj[l], s4[0]);
+        const double v = my[j] - ((s4[0] + s4[1]) + (s4[2] + s4[3]));
         const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)lo32(v), j);
         const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)hi32(v), j);
         const double d2 = mk64(lo, hi);
@@ -346,20 +353,21 @@ __global__ void __launch_bounds__(64) k_potrf_diag3(TLR t, int k, int j0, int jb
         double *wc = Wc + c * PLR;
         for (int i = 0; i < jb; i++) {
             const double *ri = Lr + i * PLR;
-            double s0 = 0.0, s1 = 0.0;
+            double s4[4] = {0.0, 0.0, 0.0, 0.0};
             int l = c;
-            for (; l + 1 < i; l += 2) {
-                s0 = fma(ri[l], wc[l], s0);
-                s1 = fma(ri[l + 1], wc[l + 1], s1);
+            for (; l + 7 < i; l += 8) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) s4[u & 3] = fma(ri[l + u], wc[l + u], s4[u & 3]);
             }
-            if (l < i) s0 = fma(ri[l], wc[l], s0);
+            for (; l < i; l++) s4[0] = fma(ri[l], wc[l], s4[0]);
             const double lii = ri[i];
             double w = 0.0;
-            if (c < jb && i >= c) w = (i == c) ? 1.0 / lii : -(s0 + s1) / lii;
+            if (c < jb && i >= c) w = (i == c) ? 1.0 / lii : -((s4[0] + s4[1]) + (s4[2] + s4[3])) / lii;
             wc[i] = w;
         }
     }
     double *Wo = Winv + (size_t)(j0 / PBK) * PBK * PBK;
+#pragma unroll 8
     for (int c = 0; c < jb; c++) {
         if (lane < jb && lane >= c) A[lane + (size_t)c * NB] = Lr[lane * PLR + c];
         if (lane < jb) Wo[lane + c * PBK] = Wc[c * PLR + lane];
