@@ -152,6 +152,25 @@ __device__ __forceinline__ double wrs8(double (&p)[8])
     return gsum<8>(keep + dppd<DPP_ROR8>(send));  // value bit 0 = lane bit 3
 }
 
+// reduce-scatter of N = 4, 8 or 16 values: lane l returns the wave total of
+// value l / (64 / N)
+template <int N>
+__device__ __forceinline__ double wrsN(double (&p)[N])
+{
+    static_assert(N == 4 || N == 8 || N == 16, "reduce-scatter width");
+    if constexpr (N == 4) return wrs4(p[0], p[1], p[2], p[3]);
+    else if constexpr (N == 8) return wrs8(p);
+    else return wrs16(p);
+}
+
+// Squared-norm floor below which a Jacobi pair is not rotated (see k_rc_wave).
+__device__ __forceinline__ double wv_skip_floor(double f2, int s, double acc, int fixed_rk)
+{
+    const double nz = (2.3e-16 * s) * (2.3e-16 * s) * f2;
+    const double tz = (fixed_rk > 0 || !(acc > 0.0)) ? 0.0 : (1e-3 * acc) * (1e-3 * acc);
+    return fmax(nz, tz);
+}
+
 // Column c of the stack: first pass from the tiles, afterwards from W.
 struct WvCols {
     const double *src0, *src1;
@@ -207,11 +226,11 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
                 for (int k = 1; k < WV_KMAX; k++)
 #pragma unroll
                     for (int c = 0; c < WV_PW; c++) p[c] = fma(a[k][jj], a[k][c], p[c]);
-                const double red = wrs4(p[0], p[1], p[2], p[3]);
+                const double red = wrsN<WV_PW>(p);
                 double S[WV_PW], prow[WV_PW];
 #pragma unroll
                 for (int c = 0; c < WV_PW; c++) {
-                    S[c] = rdlane(red, 16 * c);
+                    S[c] = rdlane(red, (64 / WV_PW) * c);
                     prow[c] = rdlane(a[0][c], j);
                 }
                 const double alpha = prow[jj], xn2 = S[jj];
@@ -292,7 +311,7 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
                     n1[k] = (nx1 && i < NB) ? col1[i] : 0.0;
                 }
             }
-            double p[8];
+            double p[2 * WV_PW];
 #pragma unroll
             for (int c = 0; c < WV_PW; c++) {
                 double a0 = v0[c] * b0[0], a1 = v0[c] * b1[0];
@@ -302,16 +321,16 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
                     a1 = fma(a[k][c], b1[k], a1);
                 }
                 p[c] = a0;
-                p[c + 4] = a1;
+                p[c + WV_PW] = a1;
             }
-            const double red = wrs8(p);
+            const double red = wrsN<2 * WV_PW>(p);
             double w0[WV_PW], w1[WV_PW];
             {
                 double d0[WV_PW], d1[WV_PW];
 #pragma unroll
                 for (int c = 0; c < WV_PW; c++) {
-                    d0[c] = rdlane(red, 8 * c);
-                    d1[c] = rdlane(red, 8 * (c + 4));
+                    d0[c] = rdlane(red, (32 / WV_PW) * c);
+                    d1[c] = rdlane(red, (32 / WV_PW) * (c + WV_PW));
                 }
 #pragma unroll
                 for (int c = 0; c < WV_PW; c++) {
@@ -509,24 +528,25 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
             double al = 0.0;
 #pragma unroll
             for (int q = 0; q < SM; q++) al = fma(xm[q], xm[q], al);
-            // rounding-noise floor of the squared column norms: a pair whose
-            // smaller column is below it is left alone (its rotation changes
-            // the larger column by less than one rounding)
-            const double nz = (2.3e-16 * s) * (2.3e-16 * s) * wsum(lane < s ? al : 0.0);
+            // A pair is left alone when its smaller column is below the
+            // rounding-noise floor (2.3e-16 s)^2 ||M||_F^2, or, when truncating
+            // at acc, below (1e-3 acc)^2: such columns stay below the
+            // truncation threshold (all of them together hold less than
+            // 0.012 acc of energy) and rotating them moves a kept column by
+            // less than one rounding of its norm.
+            const double nz = wv_skip_floor(wsum(lane < s ? al : 0.0), s, acc, fixed_rk);
             // position of column `lane` in round 0 of the round-robin schedule
             const int p0 = (lane == 0) ? 0 : lane;
             for (int sweep = 0; sweep < 60; sweep++) {
                 int rot_any = 0;
                 int pos = p0;  // column at position p in round r: ((p - 1 + r) mod (npad - 1)) + 1
                 for (int r = 0; r < npad - 1; r++) {
+                    // columns beyond s are zero, so every loop runs over all SM
+                    // entries without guards
                     if (lane < s) {
                         double *my = J + lane * Cf::LDJ;
 #pragma unroll
-                        for (int q8 = 0; q8 < SM; q8 += 8)
-                            if (q8 < s) {
-#pragma unroll
-                                for (int q = q8; q < q8 + 8; q++) my[q] = xm[q];
-                            }
+                        for (int q = 0; q < SM; q++) my[q] = xm[q];
                         my[SM] = al;
                     }
                     wv_sync();
@@ -538,14 +558,13 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                     }
                     if (lane < s && pc < s) {
                         const double *yp = J + pc * Cf::LDJ;
+                        double yv[SM];
+#pragma unroll
+                        for (int q = 0; q < SM; q++) yv[q] = yp[q];
                         const double be = yp[SM];
                         double g4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                        for (int q8 = 0; q8 < SM; q8 += 8)
-                            if (q8 < s) {
-#pragma unroll
-                                for (int q = q8; q < q8 + 8; q++) g4[q & 3] = fma(xm[q], yp[q], g4[q & 3]);
-                            }
+                        for (int q = 0; q < SM; q++) g4[q & 3] = fma(xm[q], yv[q], g4[q & 3]);
                         const double ga = (g4[0] + g4[1]) + (g4[2] + g4[3]);
                         const bool lo_ = lane < pc;
                         double cs, sn;
@@ -553,11 +572,7 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                             rot_any = 1;
                             const double ss = lo_ ? -sn : sn;
 #pragma unroll
-                            for (int q8 = 0; q8 < SM; q8 += 8)
-                                if (q8 < s) {
-#pragma unroll
-                                    for (int q = q8; q < q8 + 8; q++) xm[q] = fma(ss, yp[q], cs * xm[q]);
-                                }
+                            for (int q = 0; q < SM; q++) xm[q] = fma(ss, yv[q], cs * xm[q]);
                             // squared norm after the rotation
                             al = lo_ ? (cs * cs * al - 2.0 * cs * sn * ga + sn * sn * be)
                                      : (sn * sn * be + 2.0 * cs * sn * ga + cs * cs * al);
@@ -744,11 +759,11 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout)
                 for (int k = 2; k < WV_KMAX; k++)
 #pragma unroll
                     for (int c = 0; c < WV_PW; c++) p[c] = fma(a[k][jj], a[k][c], p[c]);
-                const double red = wrs4(p[0], p[1], p[2], p[3]);
+                const double red = wrsN<WV_PW>(p);
                 double S[WV_PW], prow[WV_PW];
 #pragma unroll
                 for (int c = 0; c < WV_PW; c++) {
-                    S[c] = rdlane(red, 16 * c);
+                    S[c] = rdlane(red, (64 / WV_PW) * c);
                     prow[c] = rdlane(ps ? a[1][c] : a[0][c], jl);
                 }
                 const double alpha = prow[jj], xn2 = S[jj];
@@ -841,10 +856,10 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout)
                 for (int k = 2; k < WV_KMAX; k++) acc = fma(a[k][c], b[k], acc);
                 p[c] = acc;
             }
-            const double red = wrs4(p[0], p[1], p[2], p[3]);
+            const double red = wrsN<WV_PW>(p);
             double d[WV_PW], w[WV_PW];
 #pragma unroll
-            for (int c = 0; c < WV_PW; c++) d[c] = rdlane(red, 16 * c);
+            for (int c = 0; c < WV_PW; c++) d[c] = rdlane(red, (64 / WV_PW) * c);
 #pragma unroll
             for (int c = 0; c < WV_PW; c++) {
                 double acc = 0.0;
@@ -984,7 +999,7 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                 const double *x = J + c * WB_LD;
                 for (int q = 0; q < s; q++) f2 = fma(x[q], x[q], f2);
             }
-            const double nz = (2.3e-16 * s) * (2.3e-16 * s) * wsum(f2);
+            const double nz = wv_skip_floor(wsum(f2), s, acc, fixed_rk);
             for (int sweep = 0; sweep < 60; sweep++) {
                 int rot_any = 0;
                 for (int r = 0; r < npad - 1; r++) {
@@ -994,28 +1009,55 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                         const int cb = (pb == 0) ? 0 : ((pb - 1 + r) % (npad - 1)) + 1;
                         if (ca < s && cb < s) {
                             double *x = J + ca * WB_LD, *y = J + cb * WB_LD;
-                            double al0 = 0.0, al1 = 0.0, be0 = 0.0, be1 = 0.0, ga0 = 0.0, ga1 = 0.0;
+                            // 8 entries per batch: 16 LDS reads in flight
+                            double a8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, b8[8] = {0, 0, 0, 0, 0, 0, 0, 0},
+                                   g8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
                             int q = 0;
-                            for (; q + 1 < s; q += 2) {
-                                const double x0 = x[q], x1 = x[q + 1], y0 = y[q], y1 = y[q + 1];
-                                al0 = fma(x0, x0, al0);
-                                al1 = fma(x1, x1, al1);
-                                be0 = fma(y0, y0, be0);
-                                be1 = fma(y1, y1, be1);
-                                ga0 = fma(x0, y0, ga0);
-                                ga1 = fma(x1, y1, ga1);
+                            for (; q + 7 < s; q += 8) {
+                                double xv[8], yv[8];
+#pragma unroll
+                                for (int u = 0; u < 8; u++) {
+                                    xv[u] = x[q + u];
+                                    yv[u] = y[q + u];
+                                }
+#pragma unroll
+                                for (int u = 0; u < 8; u++) {
+                                    a8[u] = fma(xv[u], xv[u], a8[u]);
+                                    b8[u] = fma(yv[u], yv[u], b8[u]);
+                                    g8[u] = fma(xv[u], yv[u], g8[u]);
+                                }
                             }
-                            if (q < s) {
-                                const double x0 = x[q], y0 = y[q];
-                                al0 = fma(x0, x0, al0);
-                                be0 = fma(y0, y0, be0);
-                                ga0 = fma(x0, y0, ga0);
+                            for (; q < s; q++) {
+                                const double xv = x[q], yv = y[q];
+                                a8[0] = fma(xv, xv, a8[0]);
+                                b8[0] = fma(yv, yv, b8[0]);
+                                g8[0] = fma(xv, yv, g8[0]);
                             }
-                            const double al = al0 + al1, be = be0 + be1, ga = ga0 + ga1;
+                            double al = 0.0, be = 0.0, ga = 0.0;
+#pragma unroll
+                            for (int u = 0; u < 8; u++) {
+                                al += a8[u];
+                                be += b8[u];
+                                ga += g8[u];
+                            }
                             double cs, sn;
                             if (fmin(al, be) > nz && jacobi_rot(al, be, ga, tol, cs, sn)) {
                                 rot_any = 1;
-                                for (int q2 = 0; q2 < s; q2++) {
+                                int q2 = 0;
+                                for (; q2 + 7 < s; q2 += 8) {
+                                    double xv[8], yv[8];
+#pragma unroll
+                                    for (int u = 0; u < 8; u++) {
+                                        xv[u] = x[q2 + u];
+                                        yv[u] = y[q2 + u];
+                                    }
+#pragma unroll
+                                    for (int u = 0; u < 8; u++) {
+                                        x[q2 + u] = cs * xv[u] - sn * yv[u];
+                                        y[q2 + u] = sn * xv[u] + cs * yv[u];
+                                    }
+                                }
+                                for (; q2 < s; q2++) {
                                     const double xv = x[q2], yv = y[q2];
                                     x[q2] = cs * xv - sn * yv;
                                     y[q2] = sn * xv + cs * yv;
