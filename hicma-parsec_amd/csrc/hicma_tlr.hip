@@ -686,7 +686,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
                                    (const int *)(wlist + 2 * (size_t)wcap), (const int *)(wcount + 2), wcount + 6,
                                    wws + wws_d / 2, acc, fixed_rank, maxrank, o, ctx->d_ovf);
                 wg = (int)std::min<long long>(wgrid - wres, ntask);
-                hipLaunchKernelGGL(k_rc_wide, dim3(wg), dim3(64), sizeof(double) * WB_LDS, ctx->side[1], t, k,
+                hipLaunchKernelGGL(k_rc_wide, dim3(wg), dim3(64 * WB_WAVES), sizeof(double) * WB_LDS, ctx->side[1], t, k,
                                    (const int *)(wlist + 3 * (size_t)wcap), (const int *)(wcount + 3), wcount + 7, wbs,
                                    acc, fixed_rank, maxrank, o, ctx->d_ovf);
                 // join

@@ -890,12 +890,19 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout)
 }
 
 // Tasks of step k with 64 < s <= 128 and rb <= 64 (list 3): one wave each.
-__global__ void __launch_bounds__(64, 1)
+// Four waves per workgroup: wave 0 runs every phase but the Jacobi SVD, which
+// all four share (four lanes per column pair); the workgroup already holds a
+// whole CU's LDS, so the extra waves cost no occupancy.
+#define WB_WAVES 4
+__global__ void __launch_bounds__(64 * WB_WAVES, 1)
 k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double *ws, double acc, int fixed_rk,
           int maxrank, OpLog o, int *ovf)
 {
     extern __shared__ double smem[];
-    const int lane = threadIdx.x & 63;
+    __shared__ int s_idx;
+    __shared__ double s_nz;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const bool w0 = tid < 64;
     double *L = smem;
     double *sg = L + WB_SM * WB_LD;
     const int NB = t.NB, NT = t.NT;
@@ -907,9 +914,10 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
     double *Ku = Rq + WB_SM * WB_SM;
     const int cnt = uni(*wcount);
     for (;;) {
-        int idx = 0;
-        if (lane == 0) idx = atomicAdd(wnext, 1);
-        idx = uni(idx);
+        if (tid == 0) s_idx = atomicAdd(wnext, 1);
+        __syncthreads();
+        const int idx = uni(s_idx);
+        __syncthreads();
         if (idx >= cnt) break;
         const int e = uni(wlist[idx]);
         int hi, lo;
@@ -921,7 +929,8 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
         const double *Bu = lr_tile(t, n, k), *Bv = Bu + (size_t)NB * rb;
         double *C = lr_tile(t, m, n);
         const int i16 = lane & 15, k4 = lane >> 4;
-
+        double *J = L;
+        if (w0) {
         // ---- X = Av^T Bv (ra <= 128, rb <= 64): one 16-row band of X at a time ----
         {
             double *Xl = L;  // Xl[b * WB_SM + a]
@@ -974,7 +983,6 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
         // ---- M^T = Rq Rp^T on the matrix cores (K = Rp^T = Rp's column-major
         //      storage read row-wise, staged in LDS) into W (ld s); column c of
         //      M^T (= row c of M) then goes to the Jacobi matrix ----
-        double *J = L;
         {
             for (int q = lane; q < s * WB_SM; q += 64) {
                 const int c = q % WB_SM;
@@ -990,74 +998,76 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
             }
             wv_sync();
         }
-        // ---- one-sided Jacobi, columns in LDS, one column pair per lane ----
-        {
-            const int npad = s + (s & 1), npairs = npad >> 1;
-            const double tol = 4.0e-16 * sqrt((double)s);
+            // skip floor of the Jacobi rotations
             double f2 = 0.0;
             for (int c = lane; c < s; c += 64) {
                 const double *x = J + c * WB_LD;
                 for (int q = 0; q < s; q++) f2 = fma(x[q], x[q], f2);
             }
-            const double nz = wv_skip_floor(wsum(f2), s, acc, fixed_rk);
+            const double nzv = wv_skip_floor(wsum(f2), s, acc, fixed_rk);
+            if (lane == 0) s_nz = nzv;
+        }  // wave 0
+        __syncthreads();
+        // ---- one-sided Jacobi, columns in LDS: column pair tid / 4, the four
+        //      lanes of a quad take every fourth entry (quad sums by DPP give
+        //      all four the same al, be, ga, so they agree on the rotation) ----
+        {
+            const int npad = s + (s & 1), npairs = npad >> 1;
+            const double tol = 4.0e-16 * sqrt((double)s);
+            const double nz = s_nz;
+            const int pk = tid >> 2, sub = tid & 3;
             for (int sweep = 0; sweep < 60; sweep++) {
                 int rot_any = 0;
                 for (int r = 0; r < npad - 1; r++) {
-                    if (lane < npairs) {
-                        const int pa = lane, pb = npad - 1 - lane;
-                        const int ca = (pa == 0) ? 0 : ((pa - 1 + r) % (npad - 1)) + 1;
+                    if (pk < npairs) {
+                        const int ca = (pk == 0) ? 0 : ((pk - 1 + r) % (npad - 1)) + 1;
+                        const int pb = npad - 1 - pk;
                         const int cb = (pb == 0) ? 0 : ((pb - 1 + r) % (npad - 1)) + 1;
                         if (ca < s && cb < s) {
                             double *x = J + ca * WB_LD, *y = J + cb * WB_LD;
-                            // 8 entries per batch: 16 LDS reads in flight
-                            double a8[8] = {0, 0, 0, 0, 0, 0, 0, 0}, b8[8] = {0, 0, 0, 0, 0, 0, 0, 0},
-                                   g8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-                            int q = 0;
-                            for (; q + 7 < s; q += 8) {
-                                double xv[8], yv[8];
+                            double a4[4] = {0, 0, 0, 0}, b4[4] = {0, 0, 0, 0}, g4[4] = {0, 0, 0, 0};
+                            int q = sub;
+                            for (; q + 12 < s; q += 16) {
+                                double xv[4], yv[4];
 #pragma unroll
-                                for (int u = 0; u < 8; u++) {
-                                    xv[u] = x[q + u];
-                                    yv[u] = y[q + u];
+                                for (int u = 0; u < 4; u++) {
+                                    xv[u] = x[q + 4 * u];
+                                    yv[u] = y[q + 4 * u];
                                 }
 #pragma unroll
-                                for (int u = 0; u < 8; u++) {
-                                    a8[u] = fma(xv[u], xv[u], a8[u]);
-                                    b8[u] = fma(yv[u], yv[u], b8[u]);
-                                    g8[u] = fma(xv[u], yv[u], g8[u]);
+                                for (int u = 0; u < 4; u++) {
+                                    a4[u] = fma(xv[u], xv[u], a4[u]);
+                                    b4[u] = fma(yv[u], yv[u], b4[u]);
+                                    g4[u] = fma(xv[u], yv[u], g4[u]);
                                 }
                             }
-                            for (; q < s; q++) {
+                            for (; q < s; q += 4) {
                                 const double xv = x[q], yv = y[q];
-                                a8[0] = fma(xv, xv, a8[0]);
-                                b8[0] = fma(yv, yv, b8[0]);
-                                g8[0] = fma(xv, yv, g8[0]);
+                                a4[0] = fma(xv, xv, a4[0]);
+                                b4[0] = fma(yv, yv, b4[0]);
+                                g4[0] = fma(xv, yv, g4[0]);
                             }
-                            double al = 0.0, be = 0.0, ga = 0.0;
-#pragma unroll
-                            for (int u = 0; u < 8; u++) {
-                                al += a8[u];
-                                be += b8[u];
-                                ga += g8[u];
-                            }
+                            const double al = gsum<4>((a4[0] + a4[1]) + (a4[2] + a4[3]));
+                            const double be = gsum<4>((b4[0] + b4[1]) + (b4[2] + b4[3]));
+                            const double ga = gsum<4>((g4[0] + g4[1]) + (g4[2] + g4[3]));
                             double cs, sn;
                             if (fmin(al, be) > nz && jacobi_rot(al, be, ga, tol, cs, sn)) {
                                 rot_any = 1;
-                                int q2 = 0;
-                                for (; q2 + 7 < s; q2 += 8) {
-                                    double xv[8], yv[8];
+                                int q2 = sub;
+                                for (; q2 + 12 < s; q2 += 16) {
+                                    double xv[4], yv[4];
 #pragma unroll
-                                    for (int u = 0; u < 8; u++) {
-                                        xv[u] = x[q2 + u];
-                                        yv[u] = y[q2 + u];
+                                    for (int u = 0; u < 4; u++) {
+                                        xv[u] = x[q2 + 4 * u];
+                                        yv[u] = y[q2 + 4 * u];
                                     }
 #pragma unroll
-                                    for (int u = 0; u < 8; u++) {
-                                        x[q2 + u] = cs * xv[u] - sn * yv[u];
-                                        y[q2 + u] = sn * xv[u] + cs * yv[u];
+                                    for (int u = 0; u < 4; u++) {
+                                        x[q2 + 4 * u] = cs * xv[u] - sn * yv[u];
+                                        y[q2 + 4 * u] = sn * xv[u] + cs * yv[u];
                                     }
                                 }
-                                for (; q2 < s; q2++) {
+                                for (; q2 < s; q2 += 4) {
                                     const double xv = x[q2], yv = y[q2];
                                     x[q2] = cs * xv - sn * yv;
                                     y[q2] = sn * xv + cs * yv;
@@ -1065,11 +1075,12 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                             }
                         }
                     }
-                    wv_sync();
+                    __syncthreads();
                 }
-                if (__ballot(rot_any) == 0) break;
+                if (!__syncthreads_or(rot_any)) break;
             }
         }
+        if (w0) {
         // ---- singular values (columns lane and lane + 64), order, rank ----
         for (int c = lane; c < s; c += 64) {
             const double *x = J + c * WB_LD;
@@ -1186,6 +1197,7 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
             unsigned long long words = 2ULL * NB * (ra + rc + nr) + 2ULL * NB * rb;
             atomicAdd(o.stat + 6, 8ULL * words);
         }
-        wv_sync();
+        }  // wave 0
+        __syncthreads();
     }
 }
