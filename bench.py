@@ -38,6 +38,11 @@ import hicma_tlr as H  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per recompression launch group at the default workload (N=90,000,
+# NB=900, maxrank 150, eps 1e-8), measured with two rocprofv3 --pmc passes
+# (FETCH_SIZE, WRITE_SIZE) summed over the group's kernels and scaled by the
+# counters' reading of a device copy of known size (x2.0 and x1.0).
+C2_TRAFFIC_BYTES_PER_GROUP = 6.840e9
 
 
 def log(*a):
@@ -185,7 +190,9 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-            "traffic": None,
+            "traffic": (C2_TRAFFIC_BYTES_PER_GROUP
+                        if (a.N, a.NB, a.maxrank, a.eps) == (90000, 900, 150, 1e-8) else None),
+            "traffic_unit": "bytes per launch group (HBM, rocprofv3 FETCH_SIZE + WRITE_SIZE)",
             "flops_per_launch": round(rc_ops / launches_per_step),
             "avg_launch_ms": round(kms / nl, 3) if nl else None,
             "algorithmic_GBs": round(rc_bytes / kernel_s * 1e-9, 2),
