@@ -175,3 +175,11 @@ def test_ragged_tiles():
     A = o.dense_exact()
     assert o.potrf(1e-8, 0, 130) == 0
     assert O.check_dpotrf2(o.uncompress(), A) <= 1e-7
+
+
+def test_smoke_config_compresses():
+    # __graft_entry__.smoke() factors N=1600, NB=200, maxrank 100 on the GPU;
+    # the oracle's generation must succeed there (N=1080, NB=108 does not)
+    o = O.Oracle(1600, 200, 2, 1.0, 0.1, 1e-2, 0)
+    assert o.generate(1e-8, 100, 1) == 0
+    assert O.Oracle(1080, 108, 2, 1.0, 0.1, 1e-2, 0).generate(1e-8, 54, 1) == -1
