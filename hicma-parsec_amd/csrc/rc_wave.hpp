@@ -72,17 +72,18 @@ __device__ __forceinline__ wv_d4 mfma16(double a, double b, wv_d4 c)
 
 // Tall product D (NB x nc, ld NB) = alpha * A (NB x kd) K (kd x nc) with A's
 // column c at acol(c) and K in LDS (K[c][j] at Kl[c * ldk + j], zero for
-// j >= nc).  One 16-row tile at a time, all its A loads issued up front;
+// j >= nc).  One 16-row tile at a time (tiles t0, t0 + tstep, ... when several
+// waves share the product), all its A loads issued up front;
 // the K operands are read from LDS.  Rows of a tile are read before any of
 // them is written, so D may overwrite A's storage row for row.
 template <int SM, typename ACol>
 __device__ __forceinline__ void wv_tall_mm(int NB, int kd, int nc, double alpha, ACol acol, const double *Kl,
-                                           int ldk, double *D)
+                                           int ldk, double *D, int t0 = 0, int tstep = 1)
 {
     const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
     constexpr int KS = SM / 4, TN = SM / 16;
 #pragma unroll 1
-    for (int r0 = 0; r0 < NB; r0 += 16) {
+    for (int r0 = 16 * t0; r0 < NB; r0 += 16 * tstep) {  // row tiles t0, t0 + tstep, ...
         const int row = r0 + i16;
         double av[KS];
 #pragma unroll
@@ -889,10 +890,12 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout)
     }
 }
 
-// Tasks of step k with 64 < s <= 128 and rb <= 64 (list 3): one wave each.
-// Four waves per workgroup: wave 0 runs every phase but the Jacobi SVD, which
-// all four share (four lanes per column pair); the workgroup already holds a
-// whole CU's LDS, so the extra waves cost no occupancy.
+// Tasks of step k with 64 < s <= 128 and rb <= 64 (list 3): one workgroup of
+// four waves each.  The tall products are split by 16-row tiles over the
+// waves, the two R factors run at once on waves 0 and 1, the Jacobi SVD uses
+// four lanes per column pair, and the small s x s products and solves take a
+// thread per row or column.  The workgroup already holds a whole CU's LDS,
+// so the extra waves cost no occupancy.
 #define WB_WAVES 4
 __global__ void __launch_bounds__(64 * WB_WAVES, 1)
 k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double *ws, double acc, int fixed_rk,
@@ -901,8 +904,8 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
     extern __shared__ double smem[];
     __shared__ int s_idx;
     __shared__ double s_nz;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const bool w0 = tid < 64;
+    constexpr int NTH = 64 * WB_WAVES;
+    const int tid = threadIdx.x, lane = tid & 63, wid = uni(tid >> 6);
     double *L = smem;
     double *sg = L + WB_SM * WB_LD;
     const int NB = t.NB, NT = t.NT;
@@ -930,13 +933,12 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
         double *C = lr_tile(t, m, n);
         const int i16 = lane & 15, k4 = lane >> 4;
         double *J = L;
-        if (w0) {
-        // ---- X = Av^T Bv (ra <= 128, rb <= 64): one 16-row band of X at a time ----
+        // ---- X = Av^T Bv (ra <= 128, rb <= 64): 16-row band ta of X on wave ta % 4 ----
         {
             double *Xl = L;  // Xl[b * WB_SM + a]
-            for (int q = lane; q < 64 * WB_SM; q += 64) Xl[q] = 0.0;
-            wv_sync();
-            for (int ta = 0; 16 * ta < ra; ta++) {
+            for (int q = tid; q < 64 * WB_SM; q += NTH) Xl[q] = 0.0;
+            __syncthreads();
+            for (int ta = wid; 16 * ta < ra; ta += WB_WAVES) {
                 wv_d4 xacc[4];
 #pragma unroll
                 for (int tb = 0; tb < 4; tb++) xacc[tb] = wv_d4{0.0, 0.0, 0.0, 0.0};
@@ -969,36 +971,36 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                         }
                     }
             }
-            wv_sync();
+            __syncthreads();
         }
         // ---- Qn = -Bu X^T ----
-        wv_tall_mm<WB_SM>(NB, rb, ra, -1.0, [&](int c) { return Bu + (size_t)c * NB; }, L, WB_SM, Qn);
-        wv_sync();
-        // ---- R factors ----
-        for (int side = 0; side < 2; side++) {
-            WvCols S{side ? C + (size_t)NB * rc : C, side ? (const double *)Qn : Au, rc, Wq, NB};
-            wb_geqrf_r(s, S, side ? Rq : Rp);
+        wv_tall_mm<WB_SM>(NB, rb, ra, -1.0, [&](int c) { return Bu + (size_t)c * NB; }, L, WB_SM, Qn, wid,
+                          WB_WAVES);
+        __syncthreads();
+        // ---- R factors: P = [Cu Au] on wave 0 (trailing copy in Wq), Q = [Cv Qn]
+        //      on wave 1 (trailing copy in Out, unused until the outputs) ----
+        if (wid < 2) {
+            WvCols S{wid ? C + (size_t)NB * rc : C, wid ? (const double *)Qn : Au, rc, wid ? Out : Wq, NB};
+            wb_geqrf_r(s, S, wid ? Rq : Rp);
         }
-        wv_sync();
+        __syncthreads();
         // ---- M^T = Rq Rp^T on the matrix cores (K = Rp^T = Rp's column-major
         //      storage read row-wise, staged in LDS) into W (ld s); column c of
         //      M^T (= row c of M) then goes to the Jacobi matrix ----
-        {
-            for (int q = lane; q < s * WB_SM; q += 64) {
-                const int c = q % WB_SM;
-                L[q] = (c < s) ? Rp[q] : 0.0;  // K[l][c] = Rp(c, l)
-            }
-            wv_sync();
-            wv_tall_mm<WB_SM>(s, s, s, 1.0, [&](int l) { return (const double *)Rq + (size_t)l * WB_SM; }, L, WB_SM,
-                              Wq);
-            wv_sync();
-            for (int q = lane; q < s * WB_LD; q += 64) {
-                const int c = q / WB_LD, r = q - c * WB_LD;
-                J[q] = (r < s) ? Wq[r + (size_t)c * s] : 0.0;
-            }
-            wv_sync();
+        for (int q = tid; q < s * WB_SM; q += NTH) {
+            const int c = q % WB_SM;
+            L[q] = (c < s) ? Rp[q] : 0.0;  // K[l][c] = Rp(c, l)
         }
-            // skip floor of the Jacobi rotations
+        __syncthreads();
+        wv_tall_mm<WB_SM>(s, s, s, 1.0, [&](int l) { return (const double *)Rq + (size_t)l * WB_SM; }, L, WB_SM, Wq,
+                          wid, WB_WAVES);
+        __syncthreads();
+        for (int q = tid; q < s * WB_LD; q += NTH) {
+            const int c = q / WB_LD, r = q - c * WB_LD;
+            J[q] = (r < s) ? Wq[r + (size_t)c * s] : 0.0;
+        }
+        __syncthreads();
+        if (wid == 0) {  // skip floor of the Jacobi rotations
             double f2 = 0.0;
             for (int c = lane; c < s; c += 64) {
                 const double *x = J + c * WB_LD;
@@ -1006,7 +1008,7 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
             }
             const double nzv = wv_skip_floor(wsum(f2), s, acc, fixed_rk);
             if (lane == 0) s_nz = nzv;
-        }  // wave 0
+        }
         __syncthreads();
         // ---- one-sided Jacobi, columns in LDS: column pair tid / 4, the four
         //      lanes of a quad take every fourth entry (quad sums by DPP give
@@ -1080,15 +1082,14 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                 if (!__syncthreads_or(rot_any)) break;
             }
         }
-        if (w0) {
-        // ---- singular values (columns lane and lane + 64), order, rank ----
-        for (int c = lane; c < s; c += 64) {
+        // ---- singular values, order, rank (every wave counts the same nr) ----
+        for (int c = tid; c < s; c += NTH) {
             const double *x = J + c * WB_LD;
             double v = 0.0;
             for (int q = 0; q < s; q++) v = fma(x[q], x[q], v);
             sg[c] = sqrt(v);
         }
-        wv_sync();
+        __syncthreads();
         int nr = 0;
         {
             int cntv = 0;
@@ -1096,14 +1097,14 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
             nr = (fixed_rk > 0) ? (fixed_rk < s ? fixed_rk : s) : (int)wsum((double)cntv);
         }
         if (nr > maxrank) {
-            if (lane == 0) atomicAdd(ovf, 1);
+            if (tid == 0) atomicAdd(ovf, 1);
             nr = maxrank;
         }
         nr = uni(nr);
         if (nr > 0) {
             // selected columns, normalized, to Out (column pos, ld WB_SM), then
             // row-major into LDS: Vl[l * WB_SM + pos]
-            for (int c = lane; c < s; c += 64) {
+            for (int c = tid; c < s; c += NTH) {
                 const double sc = sg[c];
                 int pos = 0;
                 for (int q = 0; q < s; q++) {
@@ -1116,75 +1117,81 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                     for (int q = 0; q < s; q++) Out[q + (size_t)pos * WB_SM] = x[q] * inv;
                 }
             }
-            wv_sync();
+            __syncthreads();
             double *Vl = L;
-            for (int q = lane; q < s * WB_SM; q += 64) {
+            for (int q = tid; q < s * WB_SM; q += NTH) {
                 const int l = q / WB_SM, j = q - l * WB_SM;
                 Vl[q] = (j < nr) ? Out[l + (size_t)j * WB_SM] : 0.0;
             }
-            wv_sync();
-            // Ku = Rq^T Vs_r: lane c computes rows c, c + 64; 32 columns at a time
-            for (int c = lane; c < s; c += 64) {
-                for (int j0 = 0; j0 < nr; j0 += 32) {
-                    double ac[32];
+            __syncthreads();
+            // Ku = Rq^T Vs_r: thread (row c, half h) takes column blocks 32 h, 32 h + 64
+            {
+                const int c = tid & (WB_SM - 1), h = tid / WB_SM;
+                if (c < s) {
+                    for (int j0 = 32 * h; j0 < nr; j0 += 64) {
+                        double ac[32];
 #pragma unroll
-                    for (int j = 0; j < 32; j++) ac[j] = 0.0;
-                    for (int l = 0; l <= c; l++) {
-                        const double rq = Rq[l + (size_t)c * WB_SM];
-                        const double *vr = Vl + l * WB_SM + j0;
+                        for (int j = 0; j < 32; j++) ac[j] = 0.0;
+                        for (int l = 0; l <= c; l++) {
+                            const double rq = Rq[l + (size_t)c * WB_SM];
+                            const double *vr = Vl + l * WB_SM + j0;
 #pragma unroll
-                        for (int j = 0; j < 32; j++) ac[j] = fma(rq, vr[j], ac[j]);
+                            for (int j = 0; j < 32; j++) ac[j] = fma(rq, vr[j], ac[j]);
+                        }
+#pragma unroll
+                        for (int j = 0; j < 32; j++)
+                            if (j0 + j < nr) Ku[c * WB_SM + j0 + j] = ac[j];
                     }
-#pragma unroll
-                    for (int j = 0; j < 32; j++)
-                        if (j0 + j < nr) Ku[c * WB_SM + j0 + j] = ac[j];
                 }
             }
-            wv_sync();
-            // Kv = Rq^-1 Vs_r in place; lane solves columns lane, lane + 64
-            double dmax = 0.0;
-            for (int q = 0; q < s; q++) dmax = fmax(dmax, fabs(Rq[q + (size_t)q * WB_SM]));
-            const double thr = 1e-14 * dmax;
-            for (int j = lane; j < nr; j += 64) {
-                for (int q = s - 1; q >= 0; q--) {
-                    double v = Vl[q * WB_SM + j];
-                    for (int l = q + 1; l < s; l++) v = fma(-Rq[q + (size_t)l * WB_SM], Vl[l * WB_SM + j], v);
-                    const double d = Rq[q + (size_t)q * WB_SM];
-                    Vl[q * WB_SM + j] = (fabs(d) > thr) ? v / d : 0.0;
+            __syncthreads();
+            // Kv = Rq^-1 Vs_r in place; thread j solves column j
+            {
+                double dmax = 0.0;
+                for (int q = 0; q < s; q++) dmax = fmax(dmax, fabs(Rq[q + (size_t)q * WB_SM]));
+                const double thr = 1e-14 * dmax;
+                for (int j = tid; j < nr; j += NTH) {
+                    for (int q = s - 1; q >= 0; q--) {
+                        double v = Vl[q * WB_SM + j];
+                        for (int l = q + 1; l < s; l++) v = fma(-Rq[q + (size_t)l * WB_SM], Vl[l * WB_SM + j], v);
+                        const double d = Rq[q + (size_t)q * WB_SM];
+                        Vl[q * WB_SM + j] = (fabs(d) > thr) ? v / d : 0.0;
+                    }
                 }
             }
-            wv_sync();
-            // ---- outputs as in k_rc_wave ----
+            __syncthreads();
+            // ---- outputs as in k_rc_wave: V' to Out + nr NB, then U' in place
+            //      over C (each wave reads and writes only its own row tiles) ----
             wv_tall_mm<WB_SM>(NB, s, nr, 1.0,
                               [&](int c) { return c < rc ? C + (size_t)(rc + c) * NB : Qn + (size_t)(c - rc) * NB; },
-                              Vl, WB_SM, Out + (size_t)nr * NB);
-            wv_sync();
-            for (int q = lane; q < s * WB_SM; q += 64) {
+                              Vl, WB_SM, Out + (size_t)nr * NB, wid, WB_WAVES);
+            __syncthreads();
+            for (int q = tid; q < s * WB_SM; q += NTH) {
                 const int c = q / WB_SM, j = q - c * WB_SM;
                 Vl[q] = (j < nr) ? Ku[c * WB_SM + j] : 0.0;
             }
-            wv_sync();
+            __syncthreads();
             wv_tall_mm<WB_SM>(
                 NB, s, nr, 1.0,
                 [&](int c) { return c < rc ? (const double *)C + (size_t)c * NB : Au + (size_t)(c - rc) * NB; }, Vl,
-                WB_SM, C);
-            wv_sync();
+                WB_SM, C, wid, WB_WAVES);
+            __syncthreads();
             {
                 const double *src = Out + (size_t)nr * NB;
                 double *d2 = C + (size_t)nr * NB;
                 const int tot = nr * NB;
-                int q = lane;
-                for (; q + 64 * 7 < tot; q += 64 * 8) {
-                    double v[8];
+                int q = tid;
+                for (; q + NTH * 3 < tot; q += NTH * 4) {
+                    double v[4];
 #pragma unroll
-                    for (int u = 0; u < 8; u++) v[u] = src[q + 64 * u];
+                    for (int u = 0; u < 4; u++) v[u] = src[q + NTH * u];
 #pragma unroll
-                    for (int u = 0; u < 8; u++) d2[q + 64 * u] = v[u];
+                    for (int u = 0; u < 4; u++) d2[q + NTH * u] = v[u];
                 }
-                for (; q < tot; q += 64) d2[q] = src[q];
+                for (; q < tot; q += NTH) d2[q] = src[q];
             }
         }
-        if (lane == 0) {
+        if (tid == 0) {
             *rank_at(t, m, n) = nr;
             unsigned long long mm = tdim(t, m), ss = s;
             unsigned long long qa = opc('q', mm, ss, 0, 0);
@@ -1197,7 +1204,6 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
             unsigned long long words = 2ULL * NB * (ra + rc + nr) + 2ULL * NB * rb;
             atomicAdd(o.stat + 6, 8ULL * words);
         }
-        }  // wave 0
         __syncthreads();
     }
 }
