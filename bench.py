@@ -42,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # NB=900, maxrank 150, eps 1e-8), measured with two rocprofv3 --pmc passes
 # (FETCH_SIZE, WRITE_SIZE) summed over the group's kernels and scaled by the
 # counters' reading of a device copy of known size (x2.0 and x1.0).
-C2_TRAFFIC_BYTES_PER_GROUP = 6.840e9
+C2_TRAFFIC_BYTES_PER_GROUP = 6.888e9
 
 
 def log(*a):
@@ -184,8 +184,9 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": ("recompression launch group of a panel step: k_classify + k_rc_wave<32|48|64> "
-                       "(one wave per low-rank gemm task: QR + Jacobi-SVD) + k_lr_gemm (stacks wider than 64)"),
+            "kernel": ("recompression launch group of a panel step: k_classify_wave + k_rc_wave<32|48|64> "
+                       "(one wave per low-rank gemm task: QR + Jacobi-SVD) + k_rc_wide (stacks of 61..128 "
+                       "columns, four waves per task), on three concurrent streams"),
             "achieved": round(achieved, 4),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",

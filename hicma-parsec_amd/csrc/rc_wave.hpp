@@ -25,6 +25,8 @@
 #define WV_SMAX 64   // largest stack width s = rc + ra, and largest rb
 #define WV_KMAX 16   // rows per lane: NB <= 64 * WV_KMAX
 #define WV_PW 4      // Householder panel width
+#define WV_S2MAX 60  // widest stack of list 2 (k_rc_wave<64>); wider stacks go to the
+                     // four-wave k_rc_wide, which balances the two side streams
 
 // Per-variant sizes: SM = largest stack width of the variant (32, 48 or 64).
 template <int SM>
@@ -381,7 +383,7 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
 
 // Tasks of step k whose update runs on one wave: both factors low rank,
 // s = rc + ra in [1, 128], rb <= WV_SMAX, NB <= 64 * WV_KMAX.  They are
-// appended to list 0 (s, rb <= 32), 1 (<= 48), 2 (<= WV_SMAX) or 3 (s <= 128,
+// appended to list 0 (s, rb <= 32), 1 (<= 48), 2 (s <= WV_S2MAX) or 3 (s <= 128,
 // rb <= WV_SMAX: k_rc_wide); the order inside
 // a list is irrelevant (tasks of one step are independent).  Listed tasks are
 // marked -1 in tasks_s; every other task keeps its k_classify value.
@@ -399,7 +401,7 @@ __global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wco
     const int s = rc + ra;
     if (ra > 0 && rb > 0 && s <= 128 && rb <= WV_SMAX) {
         tasks_s[e] = -1;
-        const int l = (s <= 32 && rb <= 32) ? 0 : ((s <= 48 && rb <= 48) ? 1 : (s <= WV_SMAX ? 2 : 3));
+        const int l = (s <= 32 && rb <= 32) ? 0 : ((s <= 48 && rb <= 48) ? 1 : (s <= WV_S2MAX ? 2 : 3));
         wlist[(size_t)l * cap + atomicAdd(wcount + l, 1)] = (int)e;
     }
 }
@@ -890,7 +892,7 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout)
     }
 }
 
-// Tasks of step k with 64 < s <= 128 and rb <= 64 (list 3): one workgroup of
+// Tasks of step k with WV_S2MAX < s <= 128 and rb <= 64 (list 3): one workgroup of
 // four waves each.  The tall products are split by 16-row tiles over the
 // waves, the two R factors run at once on waves 0 and 1, the Jacobi SVD uses
 // four lanes per column pair, and the small s x s products and solves take a
