@@ -487,27 +487,51 @@ __global__ void __launch_bounds__(BS) k_trsm_lr3(TLR t, int k, const double *Win
     double *V = lr_tile(t, m, k) + (size_t)NB * r + (size_t)c0 * NB;
     const double *L = band_tile(t, k, k);
     const int i = tid % PBK, cg = tid / PBK;  // row i, columns 4 cg .. 4 cg + 3
+    // software pipeline: the next chunk of L and X, and the block's W_b, are
+    // loaded into registers while the current chunk is multiplied from LDS
+    constexpr int LE = PBK * PBK / BS, XE = PBK * 16 / BS;
+    double pl[LE], px[XE], pw[LE];
+    auto load_chunk = [&](int b0, int nb, int l0) {
+#pragma unroll
+        for (int j = 0; j < LE; j++) {
+            const int e = tid + BS * j, ii = e % PBK, kk = e / PBK;
+            pl[j] = (ii < nb) ? L[b0 + ii + (size_t)(l0 + kk) * NB] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < XE; j++) {
+            const int e = tid + BS * j, kk = e % PBK, c = e / PBK;
+            px[j] = (c < nc) ? V[l0 + kk + (size_t)c * NB] : 0.0;
+        }
+    };
     for (int b0 = 0; b0 < NB; b0 += PBK) {
         const int nb = (NB - b0) < PBK ? (NB - b0) : PBK;
+        const double *Wb = Winv + (size_t)(b0 / PBK) * PBK * PBK;
+#pragma unroll
+        for (int j = 0; j < LE; j++) {
+            const int e = tid + BS * j, ii = e % PBK, l = e / PBK;
+            pw[j] = Wb[ii + (size_t)l * PBK];
+        }
         double acc[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int c = cg * 4 + q;
             acc[q] = (i < nb && c < nc) ? V[b0 + i + (size_t)c * NB] : 0.0;
         }
+        if (b0 > 0) load_chunk(b0, nb, 0);
         for (int l0 = 0; l0 < b0; l0 += PBK) {
-            // stage L(b0:b0+64, l0:l0+64) and X(l0:l0+64, 0:16)
-#pragma unroll 4
-            for (int e = tid; e < PBK * PBK; e += BS) {
-                const int ii = e % PBK, kk = e / PBK;
-                Ls[kk * TL3 + ii] = (ii < nb) ? L[b0 + ii + (size_t)(l0 + kk) * NB] : 0.0;
+            // L(b0:b0+64, l0:l0+64) and X(l0:l0+64, 0:16) from registers to LDS
+#pragma unroll
+            for (int j = 0; j < LE; j++) {
+                const int e = tid + BS * j, ii = e % PBK, kk = e / PBK;
+                Ls[kk * TL3 + ii] = pl[j];
             }
 #pragma unroll
-            for (int e = tid; e < PBK * 16; e += BS) {
-                const int kk = e % PBK, c = e / PBK;
-                Xs[kk * 17 + c] = (c < nc) ? V[l0 + kk + (size_t)c * NB] : 0.0;
+            for (int j = 0; j < XE; j++) {
+                const int e = tid + BS * j, kk = e % PBK, c = e / PBK;
+                Xs[kk * 17 + c] = px[j];
             }
             __syncthreads();
+            if (l0 + PBK < b0) load_chunk(b0, nb, l0 + PBK);
 #pragma unroll 8
             for (int kk = 0; kk < PBK; kk++) {
                 const double lv = Ls[kk * TL3 + i];
@@ -517,11 +541,10 @@ __global__ void __launch_bounds__(BS) k_trsm_lr3(TLR t, int k, const double *Win
             __syncthreads();
         }
         // X_b = W_b acc: W_b (lower triangular 64 x 64) into Ls, acc into Xs
-        const double *Wb = Winv + (size_t)(b0 / PBK) * PBK * PBK;
-#pragma unroll 4
-        for (int e = tid; e < PBK * PBK; e += BS) {
-            const int ii = e % PBK, l = e / PBK;
-            Ls[l * TL3 + ii] = Wb[ii + (size_t)l * PBK];
+#pragma unroll
+        for (int j = 0; j < LE; j++) {
+            const int e = tid + BS * j, ii = e % PBK, l = e / PBK;
+            Ls[l * TL3 + ii] = pw[j];
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) Xs[i * 17 + cg * 4 + q] = acc[q];
