@@ -321,50 +321,61 @@ __global__ void __launch_bounds__(64) k_potrf_diag3(TLR t, int k, int j0, int jb
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const double *my = Lr + lane * PLR;
-    for (int j = 0; j < jb; j++) {
-        // s_i = A(i, j) - sum_{l<j} L(i, l) L(j, l), i >= j
-        const double *rj = Lr + j * PLR;
-        double s4[4] = {0.0, 0.0, 0.0, 0.0};
-        int l = 0;
-        for (; l + 7 < j; l += 8) {
+    // right-looking Cholesky, lane = row: a[c] = A(lane, c) is updated in
+    // registers; column j of L is broadcast through LDS (Wc[0 .. 63] as a
+    // buffer), so each step issues up to 63 independent FMAs
+    double a[PBK];
 #pragma unroll
-            for (int u = 0; u < 8; u++) s4[u & 3] = fma(my[l + u], rj[l + u], s4[u & 3]);
+    for (int c = 0; c < PBK; c++) a[c] = Lr[lane * PLR + c];
+    double *colj = Wc;
+#pragma unroll
+    for (int j = 0; j < PBK; j++) {
+        if (j < jb) {
+            const double d2 = mk64((unsigned)__builtin_amdgcn_readlane((int)lo32(a[j]), j),
+                                   (unsigned)__builtin_amdgcn_readlane((int)hi32(a[j]), j));
+            if (!(d2 > 0.0)) {
+                if (lane == 0) atomicCAS(info, 0, k * NB + j0 + j + 1);
+                return;
+            }
+            const double d = sqrt(d2);
+            const double lij = (lane == j) ? d : ((lane > j && lane < jb) ? a[j] / d : 0.0);
+            a[j] = lij;
+            colj[lane] = lij;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int c = j + 1; c < PBK; c++) a[c] = fma(-lij, colj[c], a[c]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        for (; l < j; l++) s4[0] = fma(my[l], rj[l], s4[0]);
-        const double v = my[j] - ((s4[0] + s4[1]) + (s4[2] + s4[3]));
-        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)lo32(v), j);
-        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)hi32(v), j);
-        const double d2 = mk64(lo, hi);
-        if (!(d2 > 0.0)) {
-            if (lane == 0) atomicCAS(info, 0, k * NB + j0 + j + 1);
-            return;
+    }
+    // L (lower part of the rows) back to LDS
+#pragma unroll
+    for (int c = 0; c < PBK; c++) Lr[lane * PLR + c] = (c <= lane && lane < jb) ? a[c] : 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // W = L^-1, right-looking, lane = column c: w(l) = x(l) / L(l, l), then
+    // x(i) -= L(i, l) w(l) for i > l (L entries are broadcast LDS reads)
+    {
+#pragma unroll
+        for (int i = 0; i < PBK; i++) a[i] = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+        for (int l = 0; l < PBK; l++) {
+            if (l < jb) {
+                const double w = (lane <= l) ? a[l] / Lr[l * PLR + l] : 0.0;
+                a[l] = w;
+#pragma unroll
+                for (int i = l + 1; i < PBK; i++) a[i] = fma(-Lr[i * PLR + l], w, a[i]);
+            }
         }
-        const double d = sqrt(d2);
-        if (lane == j) Lr[lane * PLR + j] = d;
-        else if (lane > j && lane < jb) Lr[lane * PLR + j] = v / d;
+#pragma unroll
+        for (int i = 0; i < PBK; i++) Wc[lane * PLR + i] = (i < jb && lane < jb) ? a[i] : 0.0;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    // W(i, c) = (delta_ic - sum_{l=c}^{i-1} L(i, l) W(l, c)) / L(i, i), lane c
-    {
-        const int c = lane;
-        double *wc = Wc + c * PLR;
-        for (int i = 0; i < jb; i++) {
-            const double *ri = Lr + i * PLR;
-            double s4[4] = {0.0, 0.0, 0.0, 0.0};
-            int l = c;
-            for (; l + 7 < i; l += 8) {
-#pragma unroll
-                for (int u = 0; u < 8; u++) s4[u & 3] = fma(ri[l + u], wc[l + u], s4[u & 3]);
-            }
-            for (; l < i; l++) s4[0] = fma(ri[l], wc[l], s4[0]);
-            const double lii = ri[i];
-            double w = 0.0;
-            if (c < jb && i >= c) w = (i == c) ? 1.0 / lii : -((s4[0] + s4[1]) + (s4[2] + s4[3])) / lii;
-            wc[i] = w;
-        }
     }
     double *Wo = Winv + (size_t)(j0 / PBK) * PBK * PBK;
 #pragma unroll 8
