@@ -635,7 +635,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
         if (nbnd > 0)
             hipLaunchKernelGGL(k_trsm_band, dim3(ntg, nbnd), dim3(BS), 0, st, t, k, o);
         if (nlr > 0)
-            hipLaunchKernelGGL(k_trsm_lr3, dim3((cap + 15) / 16, nlr), dim3(BS), 0, st, t, k, (const double *)Winv, o);
+            hipLaunchKernelGGL(k_trsm_lr3, dim3((cap + TRSM_TC - 1) / TRSM_TC, nlr), dim3(BS), 0, st, t, k, (const double *)Winv, o);
         mark();
         HIPCHK(hipEventRecord(ctx->ev_join[2], st));
         HIPCHK(hipStreamWaitEvent(sd, ctx->ev_join[2], 0));

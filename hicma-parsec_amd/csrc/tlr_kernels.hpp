@@ -478,29 +478,32 @@ __global__ void __launch_bounds__(BS) k_trsm_lr2(TLR t, int k, const double *Win
     }
 }
 
-// low-rank tiles m = k+band .. NT-1: V <- L^{-1} V, 16 columns of one tile
+// low-rank tiles m = k+band .. NT-1: V <- L^{-1} V, TRSM_TC columns of one tile
 // per workgroup.  Row block b (64 rows) is X_b = W_b (V_b - L_b,0:b X_0:b): the
 // sum runs over 64-deep chunks of L and of the solved rows staged in LDS (so
 // every global load of a chunk is in flight at once), then the inverted
 // diagonal block W_b is applied from LDS.  grid (ceil(cap/16), count).
 #define TL3 (PBK + 1)
+#define TRSM_TC 8                   // columns of V per workgroup
+#define TRSM_XL (TRSM_TC + 1)       // LDS row stride of X
 __global__ void __launch_bounds__(BS) k_trsm_lr3(TLR t, int k, const double *Winv, OpLog o)
 {
     __shared__ double Ls[PBK * TL3];   // Ls[kk * TL3 + i] = L(b0 + i, l0 + kk); also W_b
-    __shared__ double Xs[PBK * 17];    // Xs[kk * 17 + c]  = X(l0 + kk, c); also V_b
+    __shared__ double Xs[PBK * TRSM_XL];  // Xs[kk * TRSM_XL + c] = X(l0 + kk, c); also V_b
     const int NB = t.NB, tid = threadIdx.x;
     const int m = k + t.band_size + blockIdx.y;
     const int r = *rank_at(t, m, k);
     if (blockIdx.x == 0 && tid == 0) oplog(o, t.NT, opc('t', tdim(t, m), r, 1, 0), m, k, false, m - k == 1);
-    const int c0 = blockIdx.x * 16;
+    const int c0 = blockIdx.x * TRSM_TC;
     if (c0 >= r) return;
-    const int nc = (r - c0) < 16 ? (r - c0) : 16;
+    const int nc = (r - c0) < TRSM_TC ? (r - c0) : TRSM_TC;
     double *V = lr_tile(t, m, k) + (size_t)NB * r + (size_t)c0 * NB;
     const double *L = band_tile(t, k, k);
-    const int i = tid % PBK, cg = tid / PBK;  // row i, columns 4 cg .. 4 cg + 3
+    constexpr int QC = TRSM_TC / (BS / PBK);  // columns per thread
+    const int i = tid % PBK, cg = tid / PBK;  // row i, columns QC cg .. QC cg + QC - 1
     // software pipeline: the next chunk of L and X, and the block's W_b, are
     // loaded into registers while the current chunk is multiplied from LDS
-    constexpr int LE = PBK * PBK / BS, XE = PBK * 16 / BS;
+    constexpr int LE = PBK * PBK / BS, XE = PBK * TRSM_TC / BS;
     double pl[LE], px[XE], pw[LE];
     auto load_chunk = [&](int b0, int nb, int l0) {
 #pragma unroll
@@ -522,15 +525,15 @@ __global__ void __launch_bounds__(BS) k_trsm_lr3(TLR t, int k, const double *Win
             const int e = tid + BS * j, ii = e % PBK, l = e / PBK;
             pw[j] = Wb[ii + (size_t)l * PBK];
         }
-        double acc[4];
+        double acc[QC];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int c = cg * 4 + q;
+        for (int q = 0; q < QC; q++) {
+            const int c = cg * QC + q;
             acc[q] = (i < nb && c < nc) ? V[b0 + i + (size_t)c * NB] : 0.0;
         }
         if (b0 > 0) load_chunk(b0, nb, 0);
         for (int l0 = 0; l0 < b0; l0 += PBK) {
-            // L(b0:b0+64, l0:l0+64) and X(l0:l0+64, 0:16) from registers to LDS
+            // L(b0:b0+64, l0:l0+64) and X(l0:l0+64, 0:TRSM_TC) from registers to LDS
 #pragma unroll
             for (int j = 0; j < LE; j++) {
                 const int e = tid + BS * j, ii = e % PBK, kk = e / PBK;
@@ -539,7 +542,7 @@ __global__ void __launch_bounds__(BS) k_trsm_lr3(TLR t, int k, const double *Win
 #pragma unroll
             for (int j = 0; j < XE; j++) {
                 const int e = tid + BS * j, kk = e % PBK, c = e / PBK;
-                Xs[kk * 17 + c] = px[j];
+                Xs[kk * TRSM_XL + c] = px[j];
             }
             __syncthreads();
             if (l0 + PBK < b0) load_chunk(b0, nb, l0 + PBK);
@@ -547,7 +550,7 @@ __global__ void __launch_bounds__(BS) k_trsm_lr3(TLR t, int k, const double *Win
             for (int kk = 0; kk < PBK; kk++) {
                 const double lv = Ls[kk * TL3 + i];
 #pragma unroll
-                for (int q = 0; q < 4; q++) acc[q] = fma(-lv, Xs[kk * 17 + cg * 4 + q], acc[q]);
+                for (int q = 0; q < QC; q++) acc[q] = fma(-lv, Xs[kk * TRSM_XL + cg * QC + q], acc[q]);
             }
             __syncthreads();
         }
@@ -558,17 +561,19 @@ __global__ void __launch_bounds__(BS) k_trsm_lr3(TLR t, int k, const double *Win
             Ls[l * TL3 + ii] = pw[j];
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) Xs[i * 17 + cg * 4 + q] = acc[q];
+        for (int q = 0; q < QC; q++) Xs[i * TRSM_XL + cg * QC + q] = acc[q];
         __syncthreads();
-        double x[4] = {0.0, 0.0, 0.0, 0.0};
+        double x[QC];
+#pragma unroll
+        for (int q = 0; q < QC; q++) x[q] = 0.0;
         for (int l = 0; l <= i && l < nb; l++) {
             const double w = Ls[l * TL3 + i];
 #pragma unroll
-            for (int q = 0; q < 4; q++) x[q] = fma(w, Xs[l * 17 + cg * 4 + q], x[q]);
+            for (int q = 0; q < QC; q++) x[q] = fma(w, Xs[l * TRSM_XL + cg * QC + q], x[q]);
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int c = cg * 4 + q;
+        for (int q = 0; q < QC; q++) {
+            const int c = cg * QC + q;
             if (i < nb && c < nc) V[b0 + i + (size_t)c * NB] = x[q];
         }
         __syncthreads();
