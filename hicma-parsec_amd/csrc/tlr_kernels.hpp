@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(BS) k_trsm_lr2(TLR t, int k, const double *Win
 // per workgroup.  Row block b (64 rows) is X_b = W_b (V_b - L_b,0:b X_0:b): the
 // sum runs over 64-deep chunks of L and of the solved rows staged in LDS (so
 // every global load of a chunk is in flight at once), then the inverted
-// diagonal block W_b is applied from LDS.  grid (ceil(cap/16), count).
+// diagonal block W_b is applied from LDS.  grid (ceil(cap/TRSM_TC), count).
 #define TL3 (PBK + 1)
 #define TRSM_TC 8                   // columns of V per workgroup
 #define TRSM_XL (TRSM_TC + 1)       // LDS row stride of X
