@@ -20,6 +20,9 @@
 
 // LDS scratch (doubles) that blk_gemm needs
 #define BLK_GEMM_SMEM (2 * GK * GLD)
+// sweep limit of the one-sided Jacobi SVDs; reaching it is reported as
+// not converged
+#define BLK_JACOBI_SWEEPS 60
 // LDS scratch for reductions: NWAVE * QRB + QRB
 #define BLK_RED_SMEM (NWAVE * QRB + 2 * QRB)
 
@@ -173,12 +176,6 @@ __device__ __forceinline__ void blk_copy(int rows, int cols, const double *src, 
     for (int j = 0; j < cols; j++)
         for (int i = threadIdx.x; i < rows; i += BS)
             dst[i + (size_t)j * ldd] = scale * src[i + (size_t)j * lds];
-}
-
-__device__ __forceinline__ void blk_zero(int rows, int cols, double *dst, int ldd)
-{
-    for (int j = 0; j < cols; j++)
-        for (int i = threadIdx.x; i < rows; i += BS) dst[i + (size_t)j * ldd] = 0.0;
 }
 
 // explicit unit-lower V panel (rows m x jb) from the geqrf storage at A(j0, j0)
@@ -362,6 +359,7 @@ __device__ void blk_ormqr(int m, int nc, int k, const double *A, int lda, const 
 // accumulating the right rotations in Vs (n x n, ld ldv).  On return the
 // columns of M are u_j*sigma_j, sigma[j] = ||M(:,j)||.  M and Vs may live in
 // LDS or global memory.  Round-robin pair ordering; fixed reduction order.
+// Returns the number of sweeps (BLK_JACOBI_SWEEPS: not converged).
 __device__ int blk_jacobi(int n, double *M, int ldm, double *Vs, int ldv, double *sigma,
                           int *flag, double *red)
 {
@@ -384,7 +382,7 @@ __device__ int blk_jacobi(int n, double *M, int ldm, double *Vs, int ldv, double
     const int gpb = BS / g;  // groups (pairs) handled concurrently
     const double tol = 4.0e-16 * sqrt((double)n);
     int sweep = 0;
-    for (; sweep < 60; sweep++) {
+    for (; sweep < BLK_JACOBI_SWEEPS; sweep++) {
         if (tid == 0) *flag = 0;
         __syncthreads();
         int rotated = 0;
@@ -457,32 +455,6 @@ __device__ void blk_sort_pos(int n, const double *sigma, int *pos)
         pos[j] = p;
     }
     __syncthreads();
-}
-
-// L X = B in place: L (n x n lower, non-unit, ld ldl), B (n x nc, ld ldb).
-// Blocked by GT; the diagonal block solve is column oriented.
-__device__ void blk_trsm_llnn(int n, int nc, const double *L, int ldl, double *B, int ldb, double *sm)
-{
-    const int tid = threadIdx.x;
-    for (int i0 = 0; i0 < n; i0 += GT) {
-        const int ib = (n - i0) < GT ? (n - i0) : GT;
-        for (int i = 0; i < ib; i++) {
-            const int gi = i0 + i;
-            const double d = L[gi + (size_t)gi * ldl];
-            for (int c = tid; c < nc; c += BS) B[gi + (size_t)c * ldb] /= d;
-            __syncthreads();
-            const int rows = ib - i - 1;
-            for (int e = tid; e < rows * nc; e += BS) {
-                int l = gi + 1 + e % rows, c = e / rows;
-                B[l + (size_t)c * ldb] -= L[l + (size_t)gi * ldl] * B[gi + (size_t)c * ldb];
-            }
-            __syncthreads();
-        }
-        const int rest = n - i0 - ib;
-        if (rest > 0)
-            blk_gemm<false, false>(rest, nc, ib, -1.0, L + i0 + ib + (size_t)i0 * ldl, ldl,
-                                   B + i0 + (size_t)0 * ldb, ldb, 1.0, B + i0 + ib, ldb, sm);
-    }
 }
 
 // X L^T = C in place for the rows [0, nr) of C (nr x n, ld ldc), L (n x n

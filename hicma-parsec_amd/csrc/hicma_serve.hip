@@ -1,0 +1,292 @@
+// hicma_serve.hip -- continuous batching of Gaussian log-likelihood requests.
+//
+// No counterpart in the reference.  A request asks for log det Sigma and
+// z^T Sigma^-1 z of a statistics-2d-sqexp covariance (its own N, kernel
+// parameters, point seed and vector z): generation, band auto-tuning,
+// TLR Cholesky and the forward solve, all on the device, the factor never
+// leaving HBM.  The server runs `slots` workers, each a thread with its own
+// single-stream context (one hardware queue) whose persistent grids and
+// workspaces are sized for ncu / slots CUs, so several requests' kernels run
+// on the device at once.  Requests wait in one FIFO queue and are admitted as
+// soon as a worker is free and their HBM footprint fits the budget; a request
+// whose auto-tuned band needs more memory than it reserved waits for it, and
+// no new request is admitted meanwhile (running ones only release memory, so
+// the wait ends).  Memory is allocated in stream order (hipMallocAsync), so a
+// request finishing and freeing does not stall the others.
+//
+// Every kernel's result is independent of grid sizes and of what else runs:
+// tasks are independent, reductions run in a fixed order.  A request's result
+// is therefore the same alone and in any mix.
+#include "internal.hpp"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+size_t hicma_solve_bytes(int N, int NB);  // hicma_solve.hip
+
+namespace {
+
+struct SvRequest {
+    uint64_t id;
+    hicma_lik_request r;
+    std::vector<double> z;
+    double t_submit, t_start;
+    size_t bytes;  // reserved HBM
+};
+
+double now_s()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct hicma_server {
+    int device, nslots, cu_share;
+    size_t budget, reserved;
+    hicma_ctx *probe;  // context parameters for footprint estimates
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<SvRequest *> queue;
+    std::unordered_map<uint64_t, hicma_lik_result> done;
+    uint64_t next_id;
+    int growing;   // requests waiting for band memory
+    int running;   // admitted, not finished
+    int started, init_err;
+    bool stop;
+    std::vector<std::thread> workers;
+};
+
+// HBM a request needs with a dense band of `band` tile diagonals: the
+// descriptor, the context workspace (generation / factorization / solve share
+// one buffer), the vector
+static size_t footprint(hicma_server *S, const hicma_lik_request &r, int band)
+{
+    const int cap = r.maxrank;
+    size_t ws = hicma_work_bytes(S->probe, r.N, r.NB, cap, band);
+    size_t sv = hicma_solve_bytes(r.N, r.NB);
+    return hicma_desc_bytes(r.N, r.NB, cap, band, 2) + (ws > sv ? ws : sv) + sizeof(double) * (size_t)r.N +
+           ((size_t)1 << 20);
+}
+
+static void publish(hicma_server *S, uint64_t id, const hicma_lik_result &res)
+{
+    S->done[id] = res;
+    S->cv_done.notify_all();
+}
+
+// with S->mu held: the next request to run, or null
+static SvRequest *admit(hicma_server *S)
+{
+    if (S->growing > 0) return nullptr;
+    const double t = now_s();
+    for (auto it = S->queue.begin(); it != S->queue.end();) {
+        SvRequest *q = *it;
+        if (q->bytes > S->budget) {  // can never fit
+            hicma_lik_result res{};
+            res.status = HICMA_ERR_NOMEM;
+            res.bytes = q->bytes;
+            publish(S, q->id, res);
+            delete q;
+            it = S->queue.erase(it);
+            continue;
+        }
+        if (S->reserved + q->bytes <= S->budget) {
+            S->queue.erase(it);
+            S->reserved += q->bytes;
+            S->running++;
+            q->t_start = t;
+            return q;
+        }
+        // the head has waited long enough: nothing may pass it
+        if (it == S->queue.begin() && t - q->t_submit > 1.0) return nullptr;
+        ++it;
+    }
+    return nullptr;
+}
+
+// with S->mu not held: reserve `extra` more bytes for a running request.  It
+// waits until running requests have released enough; if every running
+// request is waiting here, this one goes over the budget (its allocations
+// may then fail, which ends it with HICMA_ERR_NOMEM).
+static void grow(hicma_server *S, SvRequest *q, size_t extra)
+{
+    std::unique_lock<std::mutex> lk(S->mu);
+    S->growing++;
+    S->cv_work.notify_all();
+    S->cv_work.wait(lk, [&] { return S->reserved + extra <= S->budget || S->growing == S->running; });
+    S->reserved += extra;
+    q->bytes += extra;
+    S->growing--;
+    S->cv_work.notify_all();
+}
+
+static hicma_lik_result run(hicma_server *S, hicma_ctx *ctx, SvRequest *q)
+{
+    const hicma_lik_request &r = q->r;
+    hicma_lik_result res{};
+    res.band = 1;
+    hicma_tlr_desc *A = nullptr;
+    double *dz = nullptr;
+    int rc = 0, info = 0, band = 1;
+    rc = hicma_desc_create(ctx, &A, r.N, r.NB, r.maxrank, HICMA_KIND_SQEXP_2D, r.sigma2, r.beta, r.nugget, r.seed);
+    if (!rc) rc = STARSH_gen(ctx, HICMA_LOWER, A, A, NULL, NULL, NULL, r.eps, r.maxrank, 1, 0, &info);
+    if (!rc && info) rc = info;
+    if (!rc) {
+        const int NT = (r.N + r.NB - 1) / r.NB;
+        std::vector<int> G((size_t)NT * NT);
+        rc = parsec_rank_gather(ctx, A, G.data(), 1);
+        if (!rc) band = hicma_band_autotune(G.data(), NT, r.NB);
+        if (band < 1) rc = band;
+    }
+    if (!rc && band > 1) {
+        const size_t need = footprint(S, r, band);
+        if (need > q->bytes) grow(S, q, need - q->bytes);
+        rc = parsec_band_regenerate(ctx, HICMA_LOWER, A, NULL, NULL, NULL, band);
+    }
+    res.band = band;
+    if (!rc) {
+        int la = -1, tf = 0;
+        info = HiCMA_dpotrf_L(ctx, HICMA_LOWER, A, A, A, r.eps, 0, r.maxrank, &la, band, 300, r.maxrank, 0, &tf,
+                              NULL, NULL, NULL, 0);
+        rc = info;
+    }
+    if (!rc) rc = hicma_dev_malloc(ctx, sizeof(double) * (size_t)r.N, (void **)&dz);
+    if (!rc) rc = hicma_memcpy(ctx, dz, q->z.data(), sizeof(double) * (size_t)r.N, 1);
+    if (!rc) rc = hicma_tlr_loglik(ctx, A, dz, &res.logdet, &res.quad, &res.loglik);
+    if (dz) hicma_dev_free(ctx, dz);
+    if (A) hicma_desc_destroy(A);
+    hicma_ctx_release_ws(ctx);
+    res.status = rc;
+    res.bytes = q->bytes;
+    const double t = now_s();
+    res.t_queue = q->t_start - q->t_submit;
+    res.t_run = t - q->t_start;
+    res.t_total = t - q->t_submit;
+    return res;
+}
+
+static void worker(hicma_server *S)
+{
+    hicma_ctx *ctx = nullptr;
+    const int rc = hicma_init_ex(&ctx, S->device, HICMA_CTX_SINGLE_STREAM | HICMA_CTX_STREAM_ORDERED_ALLOC,
+                                 S->cu_share);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->started++;
+        if (rc) S->init_err = rc;
+        S->cv_done.notify_all();
+    }
+    if (rc) return;
+    for (;;) {
+        SvRequest *q = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_work.wait(lk, [&] { return (q = admit(S)) != nullptr || (S->stop && S->queue.empty()); });
+            if (!q) break;
+        }
+        const hicma_lik_result res = run(S, ctx, q);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->reserved -= q->bytes;
+            S->running--;
+            publish(S, q->id, res);
+            S->cv_work.notify_all();
+        }
+        delete q;
+    }
+    hicma_fini(ctx);
+}
+
+extern "C" int hicma_server_create(hicma_server **out, int device, int slots, double hbm_budget_gb)
+{
+    if (!out || slots < 1 || slots > 32 || !(hbm_budget_gb > 0)) return HICMA_ERR_ARG;
+    hicma_server *S = new hicma_server();
+    S->device = device;
+    S->nslots = slots;
+    S->budget = (size_t)(hbm_budget_gb * 1e9);
+    S->next_id = 1;
+    int rc = hicma_init(&S->probe, device);
+    if (rc) {
+        delete S;
+        return rc;
+    }
+    S->cu_share = S->probe->ncu / slots > 0 ? S->probe->ncu / slots : 1;
+    hicma_fini(S->probe);
+    S->probe = nullptr;
+    rc = hicma_init_ex(&S->probe, device, HICMA_CTX_SINGLE_STREAM, S->cu_share);
+    if (rc) {
+        delete S;
+        return rc;
+    }
+    for (int i = 0; i < slots; i++) S->workers.emplace_back(worker, S);
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->cv_done.wait(lk, [&] { return S->started == slots; });
+        rc = S->init_err;
+    }
+    if (rc) {
+        hicma_server_destroy(S);
+        return rc;
+    }
+    *out = S;
+    return 0;
+}
+
+extern "C" int hicma_server_submit(hicma_server *S, const hicma_lik_request *r, uint64_t *ticket)
+{
+    if (!S || !r || !ticket || r->N <= 0 || r->NB <= 0) return HICMA_ERR_ARG;
+    SvRequest *q = new SvRequest();
+    q->r = *r;
+    if (q->r.maxrank <= 0) q->r.maxrank = r->NB / 2;
+    if (q->r.maxrank > r->NB / 2 || !(q->r.eps > 0)) {
+        delete q;
+        return HICMA_ERR_ARG;
+    }
+    q->z.resize((size_t)r->N);
+    if (r->z) memcpy(q->z.data(), r->z, sizeof(double) * (size_t)r->N);
+    else hicma_normal(r->zseed, r->N, q->z.data());
+    q->r.z = nullptr;
+    q->bytes = footprint(S, q->r, 1);
+    q->t_submit = now_s();
+    std::lock_guard<std::mutex> lk(S->mu);
+    q->id = S->next_id++;
+    *ticket = q->id;
+    S->queue.push_back(q);
+    S->cv_work.notify_all();
+    return 0;
+}
+
+extern "C" int hicma_server_wait(hicma_server *S, uint64_t ticket, hicma_lik_result *res)
+{
+    if (!S || !res) return HICMA_ERR_ARG;
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (ticket == 0 || ticket >= S->next_id) return HICMA_ERR_ARG;
+    S->cv_done.wait(lk, [&] { return S->done.count(ticket) > 0; });
+    *res = S->done[ticket];
+    S->done.erase(ticket);
+    return 0;
+}
+
+extern "C" int hicma_server_destroy(hicma_server *S)
+{
+    if (!S) return 0;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+        S->cv_work.notify_all();
+    }
+    for (auto &t : S->workers) t.join();
+    if (S->probe) hicma_fini(S->probe);
+    for (SvRequest *q : S->queue) delete q;
+    delete S;
+    return 0;
+}

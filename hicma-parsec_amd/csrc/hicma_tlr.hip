@@ -3,58 +3,15 @@
 // One HIP stream per context; the PaRSEC task graph of HiCMA_dpotrf_L is
 // replaced by a panel-step schedule of batched launches whose grids cover
 // every task of a step; tile ranks never leave HBM (kernels read them).
+#include "internal.hpp"
 #include "tlr_kernels.hpp"
 #include "rc_fast.hpp"
 #include "rc_wave.hpp"
-#include "../../include/hicma_tlr.h"
 
 #include <math.h>
-#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
-#include <vector>
-
-#define HIPCHK(x)                                                                                   \
-    do {                                                                                            \
-        hipError_t e_ = (x);                                                                        \
-        if (e_ != hipSuccess) {                                                                     \
-            fprintf(stderr, "hicma_tlr: HIP error '%s' at %s:%d\n", hipGetErrorString(e_), __FILE__, \
-                    __LINE__);                                                                      \
-            return e_ == hipErrorOutOfMemory ? HICMA_ERR_NOMEM : HICMA_ERR_HIP;                     \
-        }                                                                                           \
-    } while (0)
-
-struct hicma_ctx {
-    int device;
-    int ncu;
-    hipStream_t stream;
-    double *ws;
-    size_t ws_bytes;
-    int *d_info;
-    int *d_ovf;
-    double *d_scalar;
-    size_t lds_max;
-    int profile;                     // record events around recompression launches
-    std::vector<hipEvent_t> prof_ev; // pairs
-    // the recompression variants of one panel step run concurrently on side
-    // streams (default priority), forked from and joined back to `stream`
-    // (the device exposes 4 hardware queues per process: 4 streams in all)
-    hipStream_t side[3];             // [2]: syrk(k) + potrf(k+1), one step ahead
-    hipEvent_t ev_fork, ev_join[3];
-};
-
-struct hicma_tlr_desc {
-    hicma_ctx *ctx;
-    TLR t;  // device pointers + sizes
-    int band_alloc;
-    uint64_t seed;
-    double *h_xyz;
-    unsigned long long *d_tileops;
-    unsigned long long *d_opstat;
-    unsigned long last_stats[8];
-    int last_ovf;
-};
 
 // ---------------------------------------------------------------------------
 // point set: jittered grid, Morton order (spec of the build; see DESIGN)
@@ -119,6 +76,14 @@ static void sketch(int ld, int ncol, uint64_t seed, double *om)
 }
 #define SKETCH_SEED 0x5EED0FULL
 
+// z ~ N(0, I): the same counter-based normal stream as the sketch
+extern "C" int hicma_normal(uint64_t seed, int n, double *z)
+{
+    if (n < 0 || (!z && n)) return HICMA_ERR_ARG;
+    sketch(n, 1, seed, z);
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 extern "C" unsigned long hicma_op_counts(char op, unsigned long a, unsigned long b, unsigned long c,
                                          unsigned long d)
@@ -145,7 +110,7 @@ extern "C" unsigned long hicma_op_counts(char op, unsigned long a, unsigned long
 }
 
 // ---------------------------------------------------------------------------
-extern "C" int hicma_init(hicma_ctx **ctx, int device)
+extern "C" int hicma_init_ex(hicma_ctx **ctx, int device, int flags, int cu_share)
 {
     if (!ctx) return HICMA_ERR_ARG;
     int ndev = 0;
@@ -156,32 +121,63 @@ extern "C" int hicma_init(hicma_ctx **ctx, int device)
     c->device = device;
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
-    c->ncu = prop.multiProcessorCount;
+    c->ncu_dev = prop.multiProcessorCount;
+    c->ncu = (cu_share > 0 && cu_share < c->ncu_dev) ? cu_share : c->ncu_dev;
+    c->single = (flags & HICMA_CTX_SINGLE_STREAM) != 0;
+    c->pool = (flags & HICMA_CTX_STREAM_ORDERED_ALLOC) != 0;
     c->lds_max = prop.sharedMemPerBlock;
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (int i = 0; i < 3; i++) {
-        HIPCHK(hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking));
+        if (c->single) c->side[i] = c->stream;
+        else HIPCHK(hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&c->ev_join[i], hipEventDisableTiming));
     }
     HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    if (c->pool) {
+        // keep freed blocks in the device's pool instead of returning them
+        hipMemPool_t mp;
+        HIPCHK(hipDeviceGetDefaultMemPool(&mp, device));
+        uint64_t thr = ~0ULL;
+        HIPCHK(hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &thr));
+    }
     HIPCHK(hipMalloc(&c->d_info, 4 * sizeof(int)));
+    HIPCHK(hipMemset(c->d_info, 0, 4 * sizeof(int)));
     c->d_ovf = c->d_info + 1;
     HIPCHK(hipMalloc(&c->d_scalar, 8 * sizeof(double)));
     *ctx = c;
     return 0;
 }
 
+extern "C" int hicma_init(hicma_ctx **ctx, int device) { return hicma_init_ex(ctx, device, 0, 0); }
+
+hipError_t ctx_malloc(hicma_ctx *c, void **p, size_t bytes)
+{
+    if (!c->pool) return hipMalloc(p, bytes);
+    hipError_t e = hipMallocAsync(p, bytes, c->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+}
+
+hipError_t ctx_free(hicma_ctx *c, void *p)
+{
+    if (!p) return hipSuccess;
+    return c->pool ? hipFreeAsync(p, c->stream) : hipFree(p);
+}
+
 extern "C" int hicma_fini(hicma_ctx *c)
 {
     if (!c) return 0;
     hipSetDevice(c->device);
-    if (c->ws) hipFree(c->ws);
+    if (c->ws) ctx_free(c, c->ws);
+    hipStreamSynchronize(c->stream);
     hipFree(c->d_info);
     hipFree(c->d_scalar);
     for (auto e : c->prof_ev) (void)hipEventDestroy(e);
+    for (auto e : c->cp_ev) (void)hipEventDestroy(e);
+    if (c->blas && c->blas_free) c->blas_free(c->blas);
     for (int i = 0; i < 3; i++) {
         (void)hipEventDestroy(c->ev_join[i]);
-        (void)hipStreamDestroy(c->side[i]);
+        if (!c->single) (void)hipStreamDestroy(c->side[i]);
     }
     (void)hipEventDestroy(c->ev_fork);
     hipStreamDestroy(c->stream);
@@ -189,13 +185,21 @@ extern "C" int hicma_fini(hicma_ctx *c)
     return 0;
 }
 
-static int ensure_ws(hicma_ctx *c, size_t bytes)
+int hicma_ctx_release_ws(hicma_ctx *c)
 {
-    if (c->ws_bytes >= bytes) return 0;
-    if (c->ws) HIPCHK(hipFree(c->ws));
+    if (c->ws) HIPCHK(ctx_free(c, c->ws));
     c->ws = nullptr;
     c->ws_bytes = 0;
-    HIPCHK(hipMalloc(&c->ws, bytes));
+    return 0;
+}
+
+int hicma_ensure_ws(hicma_ctx *c, size_t bytes)
+{
+    if (c->ws_bytes >= bytes) return 0;
+    if (c->ws) HIPCHK(ctx_free(c, c->ws));
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+    HIPCHK(ctx_malloc(c, (void **)&c->ws, bytes));
     c->ws_bytes = bytes;
     return 0;
 }
@@ -224,27 +228,31 @@ extern "C" int hicma_desc_create(hicma_ctx *ctx, hicma_tlr_desc **out, int N, in
     t.nugget = nugget;
     t.band_size = 1;
     A->h_xyz = (double *)malloc(sizeof(double) * (size_t)t.ndim * N);
-    hicma_points(N, t.ndim, seed, A->h_xyz);
+    A->h_xyz0 = (double *)malloc(sizeof(double) * (size_t)t.ndim * N);
+    hicma_points(N, t.ndim, seed, A->h_xyz0);
+    memcpy(A->h_xyz, A->h_xyz0, sizeof(double) * (size_t)t.ndim * N);
     double *dx = nullptr;
     int NT = t.NT;
     size_t lr_slots = (size_t)NT * (NT - 1) / 2;
-    if (hipMalloc(&dx, sizeof(double) * (size_t)t.ndim * N) != hipSuccess ||
-        hipMalloc(&t.rank, sizeof(int) * (size_t)NT * NT) != hipSuccess ||
-        (lr_slots && hipMalloc(&t.lr, sizeof(double) * lr_slots * NB * 2 * maxrank) != hipSuccess) ||
-        hipMalloc(&A->d_tileops, sizeof(unsigned long long) * (size_t)NT * NT) != hipSuccess ||
-        hipMalloc(&A->d_opstat, sizeof(unsigned long long) * 8) != hipSuccess) {
-        hipGetLastError();
-        if (dx) hipFree(dx);
-        if (t.rank) hipFree(t.rank);
-        if (t.lr) hipFree(t.lr);
-        if (A->d_tileops) hipFree(A->d_tileops);
+    if (ctx_malloc(ctx, (void **)&dx, sizeof(double) * (size_t)t.ndim * N) != hipSuccess ||
+        ctx_malloc(ctx, (void **)&t.rank, sizeof(int) * (size_t)NT * NT) != hipSuccess ||
+        (lr_slots && ctx_malloc(ctx, (void **)&t.lr, sizeof(double) * lr_slots * NB * 2 * maxrank) != hipSuccess) ||
+        ctx_malloc(ctx, (void **)&A->d_tileops, sizeof(unsigned long long) * (size_t)NT * NT) != hipSuccess ||
+        ctx_malloc(ctx, (void **)&A->d_opstat, sizeof(unsigned long long) * 16) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx_free(ctx, dx);
+        ctx_free(ctx, t.rank);
+        ctx_free(ctx, t.lr);
+        ctx_free(ctx, A->d_tileops);
         free(A->h_xyz);
+        free(A->h_xyz0);
         free(A);
         return HICMA_ERR_NOMEM;
     }
     HIPCHK(hipMemcpy(dx, A->h_xyz, sizeof(double) * (size_t)t.ndim * N, hipMemcpyHostToDevice));
     t.xyz = dx;
-    HIPCHK(hipMemset(t.rank, 0xFF, sizeof(int) * (size_t)NT * NT));
+    HIPCHK(hipMemsetAsync(t.rank, 0xFF, sizeof(int) * (size_t)NT * NT, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     *out = A;
     return 0;
 }
@@ -252,14 +260,17 @@ extern "C" int hicma_desc_create(hicma_ctx *ctx, hicma_tlr_desc **out, int N, in
 extern "C" int hicma_desc_destroy(hicma_tlr_desc *A)
 {
     if (!A) return 0;
-    hipSetDevice(A->ctx->device);
-    hipFree((void *)A->t.xyz);
-    hipFree(A->t.rank);
-    if (A->t.lr) hipFree(A->t.lr);
-    if (A->t.band) hipFree(A->t.band);
-    hipFree(A->d_tileops);
-    hipFree(A->d_opstat);
+    hicma_ctx *c = A->ctx;
+    hipSetDevice(c->device);
+    ctx_free(c, (void *)A->t.xyz);
+    ctx_free(c, A->t.rank);
+    ctx_free(c, A->t.lr);
+    ctx_free(c, A->t.band);
+    ctx_free(c, A->d_tileops);
+    ctx_free(c, A->d_opstat);
+    ctx_free(c, A->rhist);
     free(A->h_xyz);
+    free(A->h_xyz0);
     free(A);
     return 0;
 }
@@ -286,10 +297,10 @@ static int alloc_band(hicma_tlr_desc *A, int band_size)
 {
     TLR &t = A->t;
     if (A->band_alloc != band_size) {
-        if (t.band) HIPCHK(hipFree(t.band));
+        if (t.band) HIPCHK(ctx_free(A->ctx, t.band));
         t.band = nullptr;
         A->band_alloc = 0;
-        HIPCHK(hipMalloc(&t.band, sizeof(double) * (size_t)band_size * t.NT * t.NB * t.NB));
+        HIPCHK(ctx_malloc(A->ctx, (void **)&t.band, sizeof(double) * (size_t)band_size * t.NT * t.NB * t.NB));
         A->band_alloc = band_size;
     }
     t.band_size = band_size;
@@ -318,8 +329,42 @@ static int nslots_for(hicma_ctx *c, size_t slot_bytes, long long ntask)
     return (int)want;
 }
 
-extern "C" int STARSH_gen(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr_desc *Ar, double tol,
-                          int maxrank, int band_size, int send_full_tile, int *info)
+// STARSH_gen's data / kernel / index arguments (the STARS-H problem of the
+// reference, hicma_parsec/STARSH_gen.jdf:342-369): NULL keeps the
+// descriptor's own kernel parameters, kernel kind and point order; otherwise
+// they replace them.  index[i] is the point (in creation order) of matrix row i.
+static int apply_problem(hicma_tlr_desc *A, const void *data, const hicma_kernel *kernel, const hicma_index *index)
+{
+    TLR &t = A->t;
+    if (kernel) {
+        int k = *kernel;
+        if (k != HICMA_KIND_SQEXP_2D && k != HICMA_KIND_SQEXP_3D && k != HICMA_KIND_EXP_3D) return HICMA_ERR_ARG;
+        if (kind_ndim(k) != t.ndim) return HICMA_ERR_ARG;  // the point set has a fixed dimension
+        t.kind = k;
+    }
+    if (data) {
+        const hicma_kernel_params *p = (const hicma_kernel_params *)data;
+        t.sigma2 = p->sigma2;
+        t.beta = p->beta;
+        t.nugget = p->nugget;
+    }
+    if (index) {
+        const int N = t.N, nd = t.ndim;
+        std::vector<char> seen((size_t)N, 0);
+        for (int i = 0; i < N; i++) {
+            if (index[i] < 0 || index[i] >= N || seen[(size_t)index[i]]) return HICMA_ERR_ARG;
+            seen[(size_t)index[i]] = 1;
+        }
+        for (int d = 0; d < nd; d++)
+            for (int i = 0; i < N; i++) A->h_xyz[(size_t)d * N + i] = A->h_xyz0[(size_t)d * N + (size_t)index[i]];
+        HIPCHK(hipMemcpy((void *)t.xyz, A->h_xyz, sizeof(double) * (size_t)nd * N, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+extern "C" int STARSH_gen(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr_desc *Ar, void *data,
+                          hicma_kernel *kernel, hicma_index *index, double tol, int maxrank, int band_size,
+                          int send_full_tile, int *info)
 {
     (void)Ar;
     (void)send_full_tile;
@@ -327,6 +372,7 @@ extern "C" int STARSH_gen(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr
     if (uplo != HICMA_LOWER && uplo != HICMA_UPPER) return HICMA_ERR_UPLO;
     if (band_size < 1 || maxrank < 1 || maxrank > A->t.cap) return HICMA_ERR_ARG;
     HIPCHK(hipSetDevice(ctx->device));
+    RCHK(apply_problem(A, data, kernel, index));
     TLR &t = A->t;
     int rc = alloc_band(A, band_size);
     if (rc) return rc;
@@ -340,20 +386,20 @@ extern "C" int STARSH_gen(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr
         std::vector<double> om((size_t)NB * p);
         sketch(NB, p, SKETCH_SEED, om.data());
         double *d_om = nullptr;
-        HIPCHK(hipMalloc(&d_om, sizeof(double) * om.size()));
+        HIPCHK(ctx_malloc(ctx, (void **)&d_om, sizeof(double) * om.size()));
         HIPCHK(hipMemcpyAsync(d_om, om.data(), sizeof(double) * om.size(), hipMemcpyHostToDevice, ctx->stream));
         GenWs g;
         g.p = p;
         g.stride = gen_slot_doubles(NB, p, t.cap);
         int ns = nslots_for(ctx, g.stride * sizeof(double), ntask);
-        if ((rc = ensure_ws(ctx, g.stride * sizeof(double) * ns))) return rc;
+        if ((rc = hicma_ensure_ws(ctx, g.stride * sizeof(double) * ns))) return rc;
         size_t lds = sizeof(double) * (BLK_GEMM_SMEM + BLK_RED_SMEM + 2 * QRB * QRB + 3 * (size_t)p + 16);
         HIPCHK(hipFuncSetAttribute((const void *)k_compress, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_compress, dim3(ns), dim3(BS), lds, ctx->stream, t, (const double *)d_om, ctx->ws, g,
                            tol, maxrank, ctx->d_info);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        HIPCHK(hipFree(d_om));
+        HIPCHK(ctx_free(ctx, d_om));
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     int hinfo[4];
@@ -472,13 +518,15 @@ extern "C" int parsec_band_size_auto_tuning(hicma_ctx *ctx, hicma_tlr_desc *Ar, 
     return hicma_band_autotune(G, Ar->t.NT, NB);
 }
 
-extern "C" int parsec_band_regenerate(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, int band_size)
+extern "C" int parsec_band_regenerate(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, void *data, hicma_kernel *kernel,
+                                      hicma_index *index, int band_size)
 {
     if (!ctx || !A) return HICMA_ERR_ARG;
     if (uplo != HICMA_LOWER) return HICMA_ERR_UPLO;
     if (band_size < 1 || band_size > A->t.NT) return HICMA_ERR_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    RCHK(apply_problem(A, data, kernel, index));
     int rc = alloc_band(A, band_size);
     if (rc) return rc;
     if ((rc = gen_band_tiles(A))) return rc;
@@ -488,6 +536,79 @@ extern "C" int parsec_band_regenerate(hicma_ctx *ctx, int uplo, hicma_tlr_desc *
 
 // ---------------------------------------------------------------------------
 // the factorization
+
+// Workspace of one HiCMA_dpotrf_L call (doubles): syrk (Y, Z per tile row),
+// low-rank x dense gemm (W, X per pair), recompression slots of the
+// persistent grids (sized for the context's CUs), task lists.
+struct PotrfWs {
+    RcWs g;
+    int ns, nsf, wgrid, wres, wcap;
+    bool fast, wave;
+    size_t rc_lds, fstride, winv_d, tasks_d, wws_d, wbs_d, wlist_d, lrd_pairs, syrk_d, lrd_d, total;
+};
+
+static void plan_potrf_ws(hicma_ctx *ctx, int NT, int NB, int cap, int b, PotrfWs &P)
+{
+    RcWs &g = P.g;
+    g.smax = 2 * cap;
+    g.stride = rc_slot_doubles(NB, cap);
+    size_t lds_fixed = sizeof(double) * (RC_FIXED_SMEM + 2 * (size_t)g.smax + 8);
+    size_t lds_avail = ctx->lds_max > lds_fixed ? ctx->lds_max - lds_fixed : 0;
+    int jl = (int)floor(sqrt((double)(lds_avail / sizeof(double)) / 2.0));
+    if (jl > g.smax) jl = g.smax;
+    g.jlds = jl;
+    P.rc_lds = lds_fixed + sizeof(double) * 2 * (size_t)jl * jl;
+    long long maxtask = (long long)(NT - 1) * (NT - 2) / 2;
+    P.ns = nslots_for(ctx, g.stride * sizeof(double), maxtask > 0 ? maxtask : 1);
+    P.syrk_d = (size_t)NT * NB * cap + (size_t)NT * cap * cap;
+    P.lrd_pairs = (b > 1) ? (size_t)NT * (b - 1) : 0;
+    P.lrd_d = P.lrd_pairs * ((size_t)cap * NB + (size_t)cap * cap);
+    // fast recompression path (1024 threads, one row per thread)
+    P.fast = (NB <= FT);
+    P.fstride = rcf_slot_doubles(NB);
+    P.nsf = P.fast ? nslots_for(ctx, P.fstride * sizeof(double), maxtask > 0 ? maxtask : 1) : 0;
+    if (P.nsf > ctx->ncu) P.nsf = ctx->ncu;
+    P.tasks_d = ((size_t)(maxtask > 0 ? maxtask : 1) + 1) / 2 + 1;  // ints packed in doubles
+    P.winv_d = (size_t)((NB + PBK - 1) / PBK) * PBK * PBK;
+    // one-wave recompression (tasks with s <= WV_SMAX): the s <= 32 and
+    // s <= 48 variants run 4 waves per workgroup, s <= 64 two
+    P.wave = (NB <= 64 * WV_KMAX);
+    P.wgrid = P.wave ? ctx->ncu : 0;  // times WvCfg<SM>::BPC resident workgroups
+    // CUs left to the lookahead stream (syrk(k) + potrf(k+1)) while the
+    // recompression kernels of step k run
+    P.wres = (P.wave && ctx->ncu > 64 && ctx->ncu == ctx->ncu_dev) ? 16 : 0;
+    P.wcap = (int)(maxtask > 0 ? maxtask : 1);
+    // four concurrent variants (32, 48, 64 on side streams), each with up to
+    // four resident waves per CU: one quarter of the region each
+    P.wws_d = (size_t)4 * P.wgrid * 4 * wv_slot_doubles(NB);
+    P.wbs_d = P.wave ? (size_t)P.wgrid * wb_slot_doubles(NB) : 0;  // one wide-stack wave per CU
+    P.wlist_d = ((size_t)4 * P.wcap + 8) / 2 + 1;
+    P.total = g.stride * P.ns + P.syrk_d + P.lrd_d + P.fstride * P.nsf + P.tasks_d + P.winv_d + P.wws_d +
+              P.wlist_d + P.wbs_d;
+}
+
+// Device bytes a generation + factorization of this shape allocates on the
+// context, beyond the descriptor itself (the largest of the two workspaces,
+// which share one buffer, plus the generation's sketch).
+size_t hicma_work_bytes(hicma_ctx *ctx, int N, int NB, int cap, int band)
+{
+    const int NT = (N + NB - 1) / NB;
+    PotrfWs P;
+    plan_potrf_ws(ctx, NT, NB, cap, band, P);
+    int p = cap + 10;
+    if (p > NB) p = NB;
+    const long long ntask = (long long)NT * (NT - 1) / 2;
+    const size_t gs = gen_slot_doubles(NB, p, cap);
+    const size_t gen = ntask > 0 ? gs * nslots_for(ctx, gs * sizeof(double), ntask) : 0;
+    return sizeof(double) * (std::max(P.total, gen) + (size_t)NB * p);
+}
+
+size_t hicma_desc_bytes(int N, int NB, int cap, int band, int ndim)
+{
+    const size_t NT = (size_t)(N + NB - 1) / NB;
+    return sizeof(double) * (size_t)ndim * N + sizeof(int) * NT * NT + sizeof(double) * NT * (NT - 1) / 2 * NB * 2 * cap +
+           sizeof(unsigned long long) * (NT * NT + 16) + sizeof(double) * (size_t)band * NT * NB * NB;
+}
 
 // blocked Cholesky of diagonal tile k: per 64-column block, factor + invert
 // the diagonal block in LDS, panel = C W^T, lower trailing update.  The block
@@ -538,48 +659,27 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     const int NT = t.NT, NB = t.NB, cap = t.cap, b = band_size;
     HIPCHK(hipMemsetAsync(ctx->d_info, 0, 4 * sizeof(int), st));
     HIPCHK(hipMemsetAsync(A->d_tileops, 0, sizeof(unsigned long long) * (size_t)NT * NT, st));
-    HIPCHK(hipMemsetAsync(A->d_opstat, 0, sizeof(unsigned long long) * 8, st));
-    OpLog o{A->d_tileops, A->d_opstat};
+    HIPCHK(hipMemsetAsync(A->d_opstat, 0, sizeof(unsigned long long) * 16, st));
+    if (A->rhist_on) {
+        if (!A->rhist) HIPCHK(ctx_malloc(ctx, (void **)&A->rhist, sizeof(int) * 4 * (size_t)NT * NT));
+        hipLaunchKernelGGL(k_rank_hist, dim3((NT * NT + 255) / 256), dim3(256), 0, st, t, A->rhist, 0);
+    }
+    // one op log per executing stream: [8] context stream, [9] side 0,
+    // [10] side 1, [11] the lookahead stream
+    OpLog o{A->d_tileops, A->d_opstat, A->d_opstat + 8, A->rhist_on ? A->rhist : nullptr, ctx->d_info + 2};
+    OpLog o0 = o, o1 = o, od = o;
+    o0.exec += 1;
+    o1.exec += 2;
+    od.exec += 3;
 
-    // workspaces: syrk (Y, Z per row), low-rank x dense gemm (W, X per pair),
-    // recompression slots
-    RcWs g;
-    g.smax = 2 * cap;
-    g.stride = rc_slot_doubles(NB, cap);
-    size_t lds_fixed = sizeof(double) * (RC_FIXED_SMEM + 2 * (size_t)g.smax + 8);
-    size_t lds_avail = ctx->lds_max > lds_fixed ? ctx->lds_max - lds_fixed : 0;
-    int jl = (int)floor(sqrt((double)(lds_avail / sizeof(double)) / 2.0));
-    if (jl > g.smax) jl = g.smax;
-    g.jlds = jl;
-    size_t rc_lds = lds_fixed + sizeof(double) * 2 * (size_t)jl * jl;
-    long long maxtask = (long long)(NT - 1) * (NT - 2) / 2;
-    int ns = nslots_for(ctx, g.stride * sizeof(double), maxtask > 0 ? maxtask : 1);
-    size_t syrk_d = (size_t)NT * NB * cap + (size_t)NT * cap * cap;
-    size_t lrd_pairs = (b > 1) ? (size_t)NT * (b - 1) : 0;
-    size_t lrd_d = lrd_pairs * ((size_t)cap * NB + (size_t)cap * cap);
-    // fast recompression path (1024 threads, one row per thread)
-    const bool fast = (NB <= FT);
-    size_t fstride = rcf_slot_doubles(NB);
-    int nsf = fast ? nslots_for(ctx, fstride * sizeof(double), maxtask > 0 ? maxtask : 1) : 0;
-    if (nsf > ctx->ncu) nsf = ctx->ncu;
-    size_t tasks_d = ((size_t)(maxtask > 0 ? maxtask : 1) + 1) / 2 + 1;  // ints packed in doubles
-    size_t winv_d = (size_t)((NB + PBK - 1) / PBK) * PBK * PBK;
-    // one-wave recompression (tasks with s <= WV_SMAX): 2 workgroups per CU of
-    // the s <= 48 variant (4 waves each) and of the s <= 64 variant (2 waves)
-    const bool wave = (NB <= 64 * WV_KMAX);
-    const int wgrid = wave ? ctx->ncu : 0;  // times WvCfg<SM>::BPC resident workgroups
-    // CUs left to the lookahead stream (syrk(k) + potrf(k+1)) while the
-    // recompression kernels of step k run
-    const int wres = (wave && ctx->ncu > 64) ? 16 : 0;
-    const size_t wstride = wv_slot_doubles(NB);
-    const int wcap = (int)(maxtask > 0 ? maxtask : 1);
-    // four concurrent variants (32, 48, 64 on side streams), each with up to
-    // four resident waves per CU: one quarter of the region each
-    size_t wws_d = (size_t)4 * wgrid * 4 * wstride;
-    size_t wbs_d = wave ? (size_t)wgrid * wb_slot_doubles(NB) : 0;  // one wide-stack wave per CU
-    size_t wlist_d = ((size_t)4 * wcap + 8) / 2 + 1;
-    size_t total = g.stride * ns + syrk_d + lrd_d + fstride * nsf + tasks_d + winv_d + wws_d + wlist_d + wbs_d;
-    int rc = ensure_ws(ctx, total * sizeof(double));
+    PotrfWs P;
+    plan_potrf_ws(ctx, NT, NB, cap, b, P);
+    RcWs &g = P.g;
+    const int ns = P.ns, nsf = P.nsf, wgrid = P.wgrid, wres = P.wres, wcap = P.wcap;
+    const bool fast = P.fast, wave = P.wave;
+    const size_t rc_lds = P.rc_lds, fstride = P.fstride, winv_d = P.winv_d, tasks_d = P.tasks_d, wws_d = P.wws_d,
+                 wlist_d = P.wlist_d, lrd_pairs = P.lrd_pairs, total = P.total;
+    int rc = hicma_ensure_ws(ctx, total * sizeof(double));
     if (rc) return rc;
     double *rcws = ctx->ws;
     double *Yws = rcws + g.stride * ns;
@@ -611,14 +711,23 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
                                    (int)(sizeof(double) * WB_LDS)));
     }
 
-    std::vector<hipEvent_t> ev;
-    bool timing = (critical_path_time != nullptr);
-    auto mark = [&](void) {
-        if (!timing) return;
-        hipEvent_t e;
-        hipEventCreate(&e);
-        hipEventRecord(e, st);
-        ev.push_back(e);
+    // critical-path timing (hicma_parsec/HiCMA_dpotrf_L_2flow_wrapper.c:27-166):
+    // potrf(k), trsm(k+1,k) and syrk(k,k+1) durations, each measured by a
+    // pair of events on the stream that runs it (trsm of step k is one batched
+    // launch on the context stream; potrf(k+1) and syrk(k) run on the
+    // lookahead stream).  Events are created once per context and only when
+    // the caller asks for the times.
+    const bool timing = (critical_path_time != nullptr);
+    if (timing) {
+        const size_t need = 6 * (size_t)NT;
+        while (ctx->cp_ev.size() < need) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            ctx->cp_ev.push_back(e);
+        }
+    }
+    auto cp = [&](int k, int slot, hipStream_t s) -> hipError_t {
+        return timing ? hipEventRecord(ctx->cp_ev[6 * (size_t)k + slot], s) : hipSuccess;
     };
     const int ntg = (NB + GT - 1) / GT;
     const int ntl = ntg * (ntg + 1) / 2;
@@ -626,26 +735,34 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     // updates of step k only off-diagonal ones, so the former run on a side
     // stream while the latter run, and are joined back before trsm(k+1).
     hipStream_t sd = ctx->side[2];
-    if (NT > 0 && (rc = launch_potrf(A, 0, Winv, o, st))) return rc;
+    if (NT > 0) {
+        HIPCHK(cp(0, 0, st));
+        if ((rc = launch_potrf(A, 0, Winv, o, st))) return rc;
+        HIPCHK(cp(0, 1, st));
+    }
     for (int k = 0; k < NT; k++) {
-        mark();
-        mark();
         int nbnd = std::min(b - 1, NT - 1 - k);
         int nlr = NT - k - b;
+        HIPCHK(cp(k, 2, st));
         if (nbnd > 0)
             hipLaunchKernelGGL(k_trsm_band, dim3(ntg, nbnd), dim3(BS), 0, st, t, k, o);
         if (nlr > 0)
             hipLaunchKernelGGL(k_trsm_lr3, dim3((cap + TRSM_TC - 1) / TRSM_TC, nlr), dim3(BS), 0, st, t, k, (const double *)Winv, o);
-        mark();
+        HIPCHK(cp(k, 3, st));
         HIPCHK(hipEventRecord(ctx->ev_join[2], st));
         HIPCHK(hipStreamWaitEvent(sd, ctx->ev_join[2], 0));
-        if (nbnd > 0) hipLaunchKernelGGL(k_syrk_band, dim3(ntl, nbnd), dim3(BS), 0, sd, t, k, o);
+        HIPCHK(cp(k, 4, sd));
+        if (nbnd > 0) hipLaunchKernelGGL(k_syrk_band, dim3(ntl, nbnd), dim3(BS), 0, sd, t, k, od);
         if (nlr > 0) {
-            hipLaunchKernelGGL(k_syrk_lr_prep, dim3(nlr), dim3(BS), 0, sd, t, k, Yws, Zws, o);
+            hipLaunchKernelGGL(k_syrk_lr_prep, dim3(nlr), dim3(BS), 0, sd, t, k, Yws, Zws, od);
             hipLaunchKernelGGL(k_syrk_lr_update, dim3(ntl, nlr), dim3(BS), 0, sd, t, k, (const double *)Yws);
         }
-        if (k + 1 < NT && (rc = launch_potrf(A, k + 1, Winv, o, sd))) return rc;
-        mark();
+        HIPCHK(cp(k, 5, sd));
+        if (k + 1 < NT) {
+            HIPCHK(cp(k + 1, 0, sd));
+            if ((rc = launch_potrf(A, k + 1, Winv, od, sd))) return rc;
+            HIPCHK(cp(k + 1, 1, sd));
+        }
         if (nbnd >= 2)
             hipLaunchKernelGGL(k_gemm_dense, dim3(ntg * ntg, nbnd * (nbnd - 1) / 2), dim3(BS), 0, st, t, k, o);
         if (b > 1 && nlr > 0) {
@@ -675,20 +792,20 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
                 for (int i = 0; i < 2; i++) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->ev_fork, 0));
                 int wg = (int)std::min<long long>((wgrid - wres) * WvCfg<32>::BPC, (ntask + WvCfg<32>::WPB - 1) / WvCfg<32>::WPB);
                 hipLaunchKernelGGL(k_rc_wave<32>, dim3(wg), dim3(64 * WvCfg<32>::WPB), wlds32, st, t, k,
-                                   (const int *)wlist, (const int *)wcount, wcount + 4, wws, acc, fixed_rank, maxrank,
-                                   o, ctx->d_ovf);
+                                   (const int *)wlist, (const int *)wcount, wcount + 4, tasks_s, wws, acc, fixed_rank,
+                                   maxrank, o, ctx->d_ovf);
                 wg = (int)std::min<long long>((wgrid - wres) * WvCfg<48>::BPC, (ntask + WvCfg<48>::WPB - 1) / WvCfg<48>::WPB);
                 hipLaunchKernelGGL(k_rc_wave<48>, dim3(wg), dim3(64 * WvCfg<48>::WPB), wlds48, st, t, k,
-                                   (const int *)(wlist + wcap), (const int *)(wcount + 1), wcount + 5, wws + wws_d / 4,
-                                   acc, fixed_rank, maxrank, o, ctx->d_ovf);
+                                   (const int *)(wlist + wcap), (const int *)(wcount + 1), wcount + 5, tasks_s,
+                                   wws + wws_d / 4, acc, fixed_rank, maxrank, o, ctx->d_ovf);
                 wg = (int)std::min<long long>((wgrid - wres) * WvCfg<64>::BPC, (ntask + WvCfg<64>::WPB - 1) / WvCfg<64>::WPB);
                 hipLaunchKernelGGL(k_rc_wave<64>, dim3(wg), dim3(64 * WvCfg<64>::WPB), wlds64, ctx->side[0], t, k,
                                    (const int *)(wlist + 2 * (size_t)wcap), (const int *)(wcount + 2), wcount + 6,
-                                   wws + wws_d / 2, acc, fixed_rank, maxrank, o, ctx->d_ovf);
+                                   tasks_s, wws + wws_d / 2, acc, fixed_rank, maxrank, o0, ctx->d_ovf);
                 wg = (int)std::min<long long>(wgrid - wres, ntask);
                 hipLaunchKernelGGL(k_rc_wide, dim3(wg), dim3(64 * WB_WAVES), sizeof(double) * WB_LDS, ctx->side[1], t, k,
-                                   (const int *)(wlist + 3 * (size_t)wcap), (const int *)(wcount + 3), wcount + 7, wbs,
-                                   acc, fixed_rank, maxrank, o, ctx->d_ovf);
+                                   (const int *)(wlist + 3 * (size_t)wcap), (const int *)(wcount + 3), wcount + 7,
+                                   tasks_s, wbs, acc, fixed_rank, maxrank, o1, ctx->d_ovf);
                 // join
                 for (int i = 0; i < 2; i++) {
                     HIPCHK(hipEventRecord(ctx->ev_join[i], ctx->side[i]));
@@ -698,7 +815,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
             if (fast) {
                 int gf = (int)std::min<long long>(nsf, ntask);
                 hipLaunchKernelGGL(k_lr_gemm_fast, dim3(gf), dim3(FT), RCF_LDS * sizeof(double), st, t, k,
-                                   (const int *)tasks_s, fws, fstride, acc, fixed_rank, maxrank, o, ctx->d_ovf);
+                                   tasks_s, fws, fstride, acc, fixed_rank, maxrank, o, ctx->d_ovf);
             }
             hipLaunchKernelGGL(k_lr_gemm, dim3(grid), dim3(BS), rc_lds, st, t, k, (const int *)tasks_s,
                                fast ? FS : -1, rcws, g, acc, fixed_rank, maxrank, o, ctx->d_ovf);
@@ -712,34 +829,39 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
         HIPCHK(hipStreamWaitEvent(st, ctx->ev_join[2], 0));
         HIPCHK(hipGetLastError());
     }
-    mark();
+    if (A->rhist_on)
+        hipLaunchKernelGGL(k_rank_hist, dim3((NT * NT + 255) / 256), dim3(256), 0, st, t, A->rhist, 1);
     HIPCHK(hipStreamSynchronize(st));
     int hinfo[4];
     HIPCHK(hipMemcpy(hinfo, ctx->d_info, 4 * sizeof(int), hipMemcpyDeviceToHost));
-    unsigned long long stats[8];
+    unsigned long long stats[16];
     HIPCHK(hipMemcpy(stats, A->d_opstat, sizeof(stats), hipMemcpyDeviceToHost));
-    for (int i = 0; i < 8; i++) A->last_stats[i] = (unsigned long)stats[i];
+    for (int i = 0; i < 12; i++) A->last_stats[i] = (unsigned long)stats[i];
     A->last_ovf = hinfo[1];
+    A->last_unconv = hinfo[2];
     if (tileopcounters) {
         std::vector<unsigned long long> to((size_t)NT * NT);
         HIPCHK(hipMemcpy(to.data(), A->d_tileops, sizeof(unsigned long long) * to.size(), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < to.size(); i++) tileopcounters[i] = (unsigned long)to[i];
     }
+    // one executor (the device): the reference's per-thread counters sum to
+    // the total; per-stream totals are in hicma_exec_op_stats
     if (opcounters) opcounters[0] = (unsigned long)(stats[0] + stats[1]);
     if (timing) {
         double ts[3] = {0, 0, 0};
         for (int k = 0; k < NT; k++) {
             for (int ph = 0; ph < 3; ph++) {
                 float ms = 0.f;
-                hipEventElapsedTime(&ms, ev[(size_t)k * 4 + ph], ev[(size_t)k * 4 + ph + 1]);
+                HIPCHK(hipEventElapsedTime(&ms, ctx->cp_ev[6 * (size_t)k + 2 * ph], ctx->cp_ev[6 * (size_t)k + 2 * ph + 1]));
                 ts[ph] += ms * 1e-3;
             }
         }
         for (int i = 0; i < 6; i++) critical_path_time[i] = i < 3 ? ts[i] : 0.0;
-        for (auto e : ev) hipEventDestroy(e);
     }
     if (verbose && hinfo[1])
         fprintf(stderr, "hicma_tlr: %d recompressions capped at maxrank %d\n", hinfo[1], maxrank);
+    if (verbose && hinfo[2])
+        fprintf(stderr, "hicma_tlr: %d recompression SVDs stopped at the sweep limit\n", hinfo[2]);
     return hinfo[0];
 }
 
@@ -805,57 +927,60 @@ extern "C" int hicma_desc_copy(hicma_tlr_desc *dst, const hicma_tlr_desc *src)
 
 extern "C" int hicma_rank_overflows(const hicma_tlr_desc *A) { return A ? A->last_ovf : 0; }
 
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BS) k_uncompress(TLR t, int m, int n, double *out)
+extern "C" int hicma_jacobi_unconverged(const hicma_tlr_desc *A) { return A ? A->last_unconv : 0; }
+
+extern "C" int hicma_exec_op_stats(const hicma_tlr_desc *A, unsigned long *ops, int n)
 {
-    __shared__ double sm[BLK_GEMM_SMEM];
-    const int NB = t.NB;
-    int nt = (NB + GT - 1) / GT;
-    int ti = blockIdx.x % nt, tj = blockIdx.x / nt;
-    if (m - n < t.band_size) {
-        const double *T = band_tile(t, m, n);
-        for (int e = threadIdx.x; e < GT * GT; e += BS) {
-            int i = ti * GT + e % GT, j = tj * GT + e / GT;
-            if (i < NB && j < NB) out[i + (size_t)j * NB] = T[i + (size_t)j * NB];
-        }
-        return;
-    }
-    int r = *rank_at(t, m, n);
-    const double *U = lr_tile(t, m, n);
-    gemm_tile<false, true>(ti * GT, tj * GT, NB, NB, r, 1.0, U, NB, U + (size_t)NB * r, NB, 0.0, out, NB, sm);
+    if (!A || !ops || n < 0) return HICMA_ERR_ARG;
+    for (int i = 0; i < n; i++) ops[i] = i < 4 ? A->last_stats[8 + i] : 0;
+    return 0;
 }
 
-extern "C" int STARSH_check(hicma_ctx *ctx, int uplo, double *A0, hicma_tlr_desc *A, hicma_tlr_desc *Ar,
-                            int band_size, int maxrank, int *info)
+extern "C" int hicma_rank_history(hicma_tlr_desc *A, int enable)
 {
-    (void)Ar;
-    (void)maxrank;
-    (void)band_size;
-    if (!ctx || !A || !A0) return HICMA_ERR_ARG;
-    if (uplo != HICMA_LOWER) return HICMA_ERR_UPLO;
+    if (!A) return HICMA_ERR_ARG;
+    A->rhist_on = enable ? 1 : 0;
+    return 0;
+}
+
+extern "C" int hicma_get_rank_history(const hicma_tlr_desc *A, int *h)
+{
+    if (!A || !h || !A->rhist) return HICMA_ERR_ARG;
+    HIPCHK(hipSetDevice(A->ctx->device));
+    HIPCHK(hipStreamSynchronize(A->ctx->stream));
+    HIPCHK(hipMemcpy(h, A->rhist, sizeof(int) * 4 * (size_t)A->t.NT * A->t.NT, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// parsec_rank_print (hicma_parsec/rank_print.jdf:100-190): the initial,
+// minimum, maximum and final rank of every off-band tile over the last
+// factorization, and max - min; zero for band tiles, as the reference prints.
+extern "C" int parsec_rank_print(hicma_ctx *ctx, hicma_tlr_desc *Rank, int band_size)
+{
+    if (!ctx || !Rank) return HICMA_ERR_ARG;
+    if (!Rank->rhist) return HICMA_ERR_ARG;  // hicma_rank_history(Rank, 1) before HiCMA_dpotrf_L
     HIPCHK(hipSetDevice(ctx->device));
-    TLR &t = A->t;
-    const int N = t.N, NB = t.NB, NT = t.NT;
-    double *d_tile = nullptr;
-    HIPCHK(hipMalloc(&d_tile, sizeof(double) * (size_t)NB * NB));
-    std::vector<double> h((size_t)NB * NB);
-    memset(A0, 0, sizeof(double) * (size_t)N * N);
-    int nt = (NB + GT - 1) / GT;
-    for (int n = 0; n < NT; n++)
-        for (int m = n; m < NT; m++) {
-            hipLaunchKernelGGL(k_uncompress, dim3(nt * nt), dim3(BS), 0, ctx->stream, t, m, n, d_tile);
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(h.data(), d_tile, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipStreamSynchronize(ctx->stream));
-            int mm = std::min(NB, N - m * NB), nn = std::min(NB, N - n * NB);
-            for (int j = 0; j < nn; j++)
-                for (int i = 0; i < mm; i++) {
-                    size_t gi = (size_t)m * NB + i, gj = (size_t)n * NB + j;
-                    if (gi >= gj) A0[gi + gj * N] = h[i + (size_t)j * NB];
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int NT = Rank->t.NT;
+    std::vector<int> h(4 * (size_t)NT * NT);
+    HIPCHK(hipMemcpy(h.data(), Rank->rhist, sizeof(int) * h.size(), hipMemcpyDeviceToHost));
+    const char *title[5] = {"Initial Rank", "Minimum Rank", "Maximum Rank", "Final Rank", "Difference Rank, max - min"};
+    for (int q = 0; q < 5; q++) {
+        printf("\n%s\n", title[q]);
+        for (int i = 0; i < NT; i++) {
+            for (int j = 0; j < NT; j++) {
+                int v = 0;
+                if (i - j >= band_size) {
+                    const int *p = &h[4 * ((size_t)j * NT + i)];
+                    v = q < 4 ? p[q] : p[2] - p[1];
                 }
+                printf("%3d ", v);
+            }
+            printf("\n");
         }
-    HIPCHK(hipFree(d_tile));
-    if (info) *info = 0;
+        printf("\n\n");
+    }
+    fflush(stdout);
     return 0;
 }
 

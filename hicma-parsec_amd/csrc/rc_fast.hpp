@@ -9,9 +9,11 @@
 // Algorithm (same mathematics as HCORE's QR + SVD recompression):
 //   P = [Cu Au], Q = [Cv Qn];  R factors Rp, Rq by Householder QR;
 //   M = Rp Rq^T = Us S Vs^T (one-sided Jacobi);  keep r' columns;
-//   U' = Qp Us_r S_r = P (Rq^T Vs_r),   V' = Qq Vs_r = Q (Rp^T Us_r S_r^-1).
-// The last two identities follow from P = Qp Rp, Q = Qq Rq and
-// M Vs_r = Us_r S_r, so neither Householder basis is ever applied.
+//   U' = Qp Us_r S_r = P (Rq^T Vs_r),   V' = Qq Vs_r = Q (Rq^-1 Vs_r).
+// The identities follow from P = Qp Rp, Q = Qq Rq and M Vs_r = Us_r S_r, so
+// neither Householder basis is ever applied.  The solve with Rq is a plain
+// back substitution with an a-posteriori error bound (see k_rc_wave); a task
+// that fails it is handed to k_lr_gemm.
 #pragma once
 #include "tlr_kernels.hpp"
 
@@ -143,9 +145,11 @@ __device__ __forceinline__ void f_geqrf_r(int m, int n, double *A, int lda, doub
 }
 
 // ---- one-sided Jacobi on an n x n matrix in LDS ---------------------------
-__device__ __forceinline__ void f_jacobi(int n, double *M, int ldm, double *Vs, int ldv, double *sigma, int *flag,
+// returns the number of sweeps (BLK_JACOBI_SWEEPS: not converged)
+__device__ __forceinline__ int f_jacobi(int n, double *M, int ldm, double *Vs, int ldv, double *sigma, int *flag,
                          double *red1)
 {
+    int sweep = 0;
     const int tid = threadIdx.x;
     for (int e = tid; e < n * n; e += FT) {
         int r = e % n, c = e / n;
@@ -156,7 +160,7 @@ __device__ __forceinline__ void f_jacobi(int n, double *M, int ldm, double *Vs, 
         const int npad = n + (n & 1), npairs = npad / 2;
         constexpr int g = 8;  // lanes per column pair (n <= FS: at most 8 rows per lane)
         const double tol = 4.0e-16 * sqrt((double)n);
-        for (int sweep = 0; sweep < 60; sweep++) {
+        for (; sweep < BLK_JACOBI_SWEEPS; sweep++) {
             if (tid == 0) *flag = 0;
             __syncthreads();
             int rotated = 0;
@@ -211,6 +215,7 @@ __device__ __forceinline__ void f_jacobi(int n, double *M, int ldm, double *Vs, 
         sigma[tid] = sqrt(p);
     }
     __syncthreads();
+    return sweep;
 }
 
 __host__ __device__ inline size_t rcf_slot_doubles(int NB) { return (size_t)NB * FS * 5 + 64; }
@@ -243,7 +248,7 @@ __global__ void k_classify(TLR t, int k, int *tasks_s)
 }
 
 // fast recompression: tasks with 0 <= s <= FS and NB <= FT
-__global__ void __launch_bounds__(FT) k_lr_gemm_fast(TLR t, int k, const int *tasks_s, double *ws, size_t stride,
+__global__ void __launch_bounds__(FT) k_lr_gemm_fast(TLR t, int k, int *tasks_s, double *ws, size_t stride,
                                                      double acc, int fixed_rk, int maxrank, OpLog o, int *ovf)
 {
     extern __shared__ double sm[];
@@ -377,7 +382,9 @@ __global__ void __launch_bounds__(FT) k_lr_gemm_fast(TLR t, int k, const int *ta
                 M[r + c * FS] = v;
             }
             __syncthreads();
-            f_jacobi(s, M, FS, Vs, FS, sig, flag, sc + FW * FPB + 2 * FPB + FPB * FPB);
+            if (f_jacobi(s, M, FS, Vs, FS, sig, flag, sc + FW * FPB + 2 * FPB + FPB * FPB) >= BLK_JACOBI_SWEEPS &&
+                tid == 0)
+                atomicAdd(o.unconv, 1);
             for (int j = tid; j < s; j += FT) {
                 double sj = sig[j];
                 int p = 0;
@@ -398,34 +405,54 @@ __global__ void __launch_bounds__(FT) k_lr_gemm_fast(TLR t, int k, const int *ta
             }
             __syncthreads();
             nr = *nrs;
-            // Ku = Rq^T Vs_sel (s x nr) into R2's columns, Kv = Rp^T M_sel S^-2 into R1's columns
+            // Ku = Rq^T Vs_sel (s x nr) into R2's columns once Rq is no longer
+            // read; Kv = Rq^-1 Vs_sel by back substitution into R1's columns
+            // (Rp is no longer needed), stored transposed: K[pos + c * FS]
             {
-                double ku[4], kv[4];
+                double ku[4];
                 int cnt = 0;
                 for (int e = tid; e < s * s && cnt < 4; e += FT, cnt++) {
                     int c = e % s, j = e / s;  // output row c, source column j
                     ku[cnt] = 0.0;
-                    kv[cnt] = 0.0;
                     if (pos[j] < nr) {
-                        double su = 0.0, sv = 0.0;
-                        for (int l = 0; l <= c; l++) {
-                            su = fma(R2[l + c * FS], Vs[l + j * FS], su);
-                            sv = fma(R1[l + c * FS], M[l + j * FS], sv);
-                        }
-                        double sg = sig[j];
+                        double su = 0.0;
+                        for (int l = 0; l <= c; l++) su = fma(R2[l + c * FS], Vs[l + j * FS], su);
                         ku[cnt] = su;
-                        kv[cnt] = (sg > 0.0) ? sv / (sg * sg) : 0.0;
                     }
                 }
-                __syncthreads();
+                int bad = 0;
+                if (tid < s && pos[tid] < nr) {
+                    const int j = tid, pj = pos[tid];
+                    double fro2 = 0.0;
+                    for (int q = 0; q < s; q++) fro2 = fma(sig[q], sig[q], fro2);
+                    double gmax = 0.0;
+                    for (int q = s - 1; q >= 0; q--) {
+                        double v = Vs[q + j * FS], gq = 0.0;
+                        for (int l = q + 1; l < s; l++) {
+                            const double rql = R2[q + l * FS], xl = R1[pj + l * FS];
+                            v = fma(-rql, xl, v);
+                            gq = fma(fabs(rql), fabs(xl), gq);
+                        }
+                        const double d = R2[q + q * FS];
+                        const bool ok = fabs(d) > 1e-300;
+                        const double x = ok ? v / d : 0.0;
+                        if (!ok) bad = 1;
+                        gmax = fmax(gmax, fma(fabs(d), fabs(x), gq));
+                        R1[pj + q * FS] = x;
+                    }
+                    if (!(sig[j] * s * 2.3e-16 * gmax <= rc_solve_tol(acc, fixed_rk, sqrt(fro2)))) bad = 1;
+                }
+                if (__syncthreads_or(bad)) {
+                    // C untouched: the Householder path redoes the task
+                    if (tid == 0) tasks_s[task] = RC_REDO;
+                    __syncthreads();
+                    continue;
+                }
                 cnt = 0;
                 for (int e = tid; e < s * s && cnt < 4; e += FT, cnt++) {
                     int c = e % s, j = e / s;
-                    if (pos[j] < nr) {
-                        // row c, output column pos[j], stored transposed in column c
-                        R2[pos[j] + c * FS] = ku[cnt];
-                        R1[pos[j] + c * FS] = kv[cnt];
-                    }
+                    // row c, output column pos[j], stored transposed in column c
+                    if (pos[j] < nr) R2[pos[j] + c * FS] = ku[cnt];
                 }
                 __syncthreads();
             }
@@ -454,6 +481,7 @@ __global__ void __launch_bounds__(FT) k_lr_gemm_fast(TLR t, int k, const int *ta
         }
         if (tid == 0) {
             *rank_at(t, m, n) = nr;
+            rank_hist(o, NT, m, n, nr);
             unsigned long long mm = tdim(t, m), ss = s;
             unsigned long long qa = opc('q', mm, ss, 0, 0), qb;
             if (bdense) qb = opc('q', mm, ss, 0, 0) + opc('m', ra, mm, mm, 0);

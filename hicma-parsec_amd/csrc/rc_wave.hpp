@@ -28,6 +28,7 @@
 #define WV_S2MAX 60  // widest stack of list 2 (k_rc_wave<64>); wider stacks go to the
                      // four-wave k_rc_wide, which balances the two side streams
 
+
 // Per-variant sizes: SM = largest stack width of the variant (32, 48 or 64).
 template <int SM>
 struct WvCfg {
@@ -43,15 +44,6 @@ struct WvCfg {
 __host__ __device__ inline size_t wv_slot_doubles(int NB)
 {
     return (size_t)NB * 4 * WV_SMAX + 4 * (size_t)WV_SMAX * WV_SMAX;
-}
-
-// LDS ordering inside one wave: a code-motion barrier plus wavefront-scope
-// fences (LDS operations of one wave complete in order)
-__device__ __forceinline__ void wv_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ double rdlane(double v, int l)
@@ -408,8 +400,8 @@ __global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wco
 
 template <int SM>
 __global__ void __launch_bounds__(64 * WvCfg<SM>::WPB, WvCfg<SM>::MINW)
-k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double *ws, double acc, int fixed_rk,
-          int maxrank, OpLog o, int *ovf)
+k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *tasks_s, double *ws, double acc,
+          int fixed_rk, int maxrank, OpLog o, int *ovf)
 {
     using Cf = WvCfg<SM>;
     extern __shared__ double smem[];
@@ -524,6 +516,7 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
             wv_sync();
         }
         // ---- one-sided Jacobi on the columns of M^T (lane c owns column c) ----
+        double fro = 0.0;  // ||M||_F
         {
             double *J = L;
             const int npad = s + (s & 1);
@@ -537,10 +530,13 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
             // truncation threshold (all of them together hold less than
             // 0.012 acc of energy) and rotating them moves a kept column by
             // less than one rounding of its norm.
-            const double nz = wv_skip_floor(wsum(lane < s ? al : 0.0), s, acc, fixed_rk);
+            const double f2 = wsum(lane < s ? al : 0.0);
+            fro = sqrt(f2);
+            const double nz = wv_skip_floor(f2, s, acc, fixed_rk);
             // position of column `lane` in round 0 of the round-robin schedule
             const int p0 = (lane == 0) ? 0 : lane;
-            for (int sweep = 0; sweep < 60; sweep++) {
+            int sweep = 0;
+            for (; sweep < BLK_JACOBI_SWEEPS; sweep++) {
                 int rot_any = 0;
                 int pos = p0;  // column at position p in round r: ((p - 1 + r) mod (npad - 1)) + 1
                 for (int r = 0; r < npad - 1; r++) {
@@ -591,6 +587,7 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                 for (int q = 0; q < SM; q++) al = fma(xm[q], xm[q], al);
                 if (__ballot(rot_any) == 0) break;
             }
+            if (sweep >= BLK_JACOBI_SWEEPS && lane == 0) atomicAdd(o.unconv, 1);
         }
         // ---- singular values, order, rank ----
         double sig = 0.0;
@@ -614,13 +611,15 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
         nr = uni(nr);
         wv_sync();
         if (nr > 0) {
-            // Vs_r into LDS, row-major: Vl[l * SM + pos]
+            // Vs_r into LDS, row-major: Vl[l * SM + pos]; sigma by position
             double *Vl = L;
+            double *sgp = sg + SM;
             if (lane < s && pos < nr) {
                 const double inv = sig > 0.0 ? 1.0 / sig : 0.0;
 #pragma unroll
                 for (int q = 0; q < SM; q++)
                     if (q < s) Vl[q * SM + pos] = xm[q] * inv;
+                sgp[pos] = sig;
             }
             wv_sync();
             // Ku = Rq^T Vs_r: lane c computes row c -> Ku[c * SM + j]
@@ -645,18 +644,38 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                 }
             }
             wv_sync();
-            // Kv = Rq^-1 Vs_r in place (lane j solves column j); pivots below
-            // 1e-14 of the largest give a zero component (exactly singular Rq)
-            double dmax = 0.0;
-            for (int q = 0; q < s; q++) dmax = fmax(dmax, fabs(Rq[q * SM + q]));
-            const double thr = 1e-14 * dmax;
+            // Kv = Rq^-1 Vs_r in place (lane j solves column j) by plain back
+            // substitution.  Q Kv = Qq Vs_r for any Kv with Rq Kv = Vs_r, and
+            // the substitution is backward stable, so the tiny pivots of a
+            // numerically rank-deficient Q are divided by, not dropped (dropping
+            // one breaks the solve when its row carries other entries).  Column
+            // j then errs in the tile by at most
+            // sigma_j s eps max_q sum_l |Rq(q,l)| |Kv(l,j)|; a task over the
+            // tolerance, or with an exactly zero pivot, is left untouched and
+            // handed to the Householder path (k_lr_gemm).
+            bool bad = false;
             if (lane < nr) {
+                double gmax = 0.0;
                 for (int q = s - 1; q >= 0; q--) {
-                    double v = Vl[q * SM + lane];
-                    for (int l = q + 1; l < s; l++) v = fma(-Rq[q * SM + l], Vl[l * SM + lane], v);
+                    double v = Vl[q * SM + lane], gq = 0.0;
+                    for (int l = q + 1; l < s; l++) {
+                        const double rql = Rq[q * SM + l], xl = Vl[l * SM + lane];
+                        v = fma(-rql, xl, v);
+                        gq = fma(fabs(rql), fabs(xl), gq);
+                    }
                     const double d = Rq[q * SM + q];
-                    Vl[q * SM + lane] = (fabs(d) > thr) ? v / d : 0.0;
+                    const bool ok = fabs(d) > 1e-300;
+                    const double x = ok ? v / d : 0.0;
+                    bad = bad || !ok;
+                    gmax = fmax(gmax, fma(fabs(d), fabs(x), gq));
+                    Vl[q * SM + lane] = x;
                 }
+                bad = bad || !(sgp[lane] * s * 2.3e-16 * gmax <= rc_solve_tol(acc, fixed_rk, fro));
+            }
+            if (__ballot(bad) != 0) {
+                if (lane == 0) tasks_s[e] = RC_REDO;
+                wv_sync();
+                continue;
             }
             wv_sync();
             // ---- outputs.  V' = Q Kv first (Kv in LDS) into Out; then Ku is
@@ -694,6 +713,7 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
         }
         if (lane == 0) {
             *rank_at(t, m, n) = nr;
+            rank_hist(o, NT, m, n, nr);
             unsigned long long mm = tdim(t, m), ss = s;
             unsigned long long qa = opc('q', mm, ss, 0, 0);
             unsigned long long qb = opc('m', ra, rb, mm, 0) + opc('m', ra, mm, rb, 0) + opc('q', mm, ss, 0, 0);
@@ -900,12 +920,12 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout)
 // so the extra waves cost no occupancy.
 #define WB_WAVES 4
 __global__ void __launch_bounds__(64 * WB_WAVES, 1)
-k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double *ws, double acc, int fixed_rk,
-          int maxrank, OpLog o, int *ovf)
+k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *tasks_s, double *ws, double acc,
+          int fixed_rk, int maxrank, OpLog o, int *ovf)
 {
     extern __shared__ double smem[];
     __shared__ int s_idx;
-    __shared__ double s_nz;
+    __shared__ double s_nz, s_fro;
     constexpr int NTH = 64 * WB_WAVES;
     const int tid = threadIdx.x, lane = tid & 63, wid = uni(tid >> 6);
     double *L = smem;
@@ -1008,8 +1028,12 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                 const double *x = J + c * WB_LD;
                 for (int q = 0; q < s; q++) f2 = fma(x[q], x[q], f2);
             }
-            const double nzv = wv_skip_floor(wsum(f2), s, acc, fixed_rk);
-            if (lane == 0) s_nz = nzv;
+            const double f2t = wsum(f2);
+            const double nzv = wv_skip_floor(f2t, s, acc, fixed_rk);
+            if (lane == 0) {
+                s_nz = nzv;
+                s_fro = sqrt(f2t);
+            }
         }
         __syncthreads();
         // ---- one-sided Jacobi, columns in LDS: column pair tid / 4, the four
@@ -1020,7 +1044,8 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
             const double tol = 4.0e-16 * sqrt((double)s);
             const double nz = s_nz;
             const int pk = tid >> 2, sub = tid & 3;
-            for (int sweep = 0; sweep < 60; sweep++) {
+            int sweep = 0;
+            for (; sweep < BLK_JACOBI_SWEEPS; sweep++) {
                 int rot_any = 0;
                 for (int r = 0; r < npad - 1; r++) {
                     if (pk < npairs) {
@@ -1083,6 +1108,7 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                 }
                 if (!__syncthreads_or(rot_any)) break;
             }
+            if (sweep >= BLK_JACOBI_SWEEPS && tid == 0) atomicAdd(o.unconv, 1);
         }
         // ---- singular values, order, rank (every wave counts the same nr) ----
         for (int c = tid; c < s; c += NTH) {
@@ -1117,6 +1143,7 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                     const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
                     const double *x = J + c * WB_LD;
                     for (int q = 0; q < s; q++) Out[q + (size_t)pos * WB_SM] = x[q] * inv;
+                    sg[WB_SM + pos] = sc;
                 }
             }
             __syncthreads();
@@ -1147,21 +1174,35 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
                 }
             }
             __syncthreads();
-            // Kv = Rq^-1 Vs_r in place; thread j solves column j
+            // Kv = Rq^-1 Vs_r in place, thread j solves column j: plain back
+            // substitution with the error bound and hand-off of k_rc_wave
+            int bad = 0;
             {
-                double dmax = 0.0;
-                for (int q = 0; q < s; q++) dmax = fmax(dmax, fabs(Rq[q + (size_t)q * WB_SM]));
-                const double thr = 1e-14 * dmax;
+                const double tol = rc_solve_tol(acc, fixed_rk, s_fro);
                 for (int j = tid; j < nr; j += NTH) {
+                    double gmax = 0.0;
                     for (int q = s - 1; q >= 0; q--) {
-                        double v = Vl[q * WB_SM + j];
-                        for (int l = q + 1; l < s; l++) v = fma(-Rq[q + (size_t)l * WB_SM], Vl[l * WB_SM + j], v);
+                        double v = Vl[q * WB_SM + j], gq = 0.0;
+                        for (int l = q + 1; l < s; l++) {
+                            const double rql = Rq[q + (size_t)l * WB_SM], xl = Vl[l * WB_SM + j];
+                            v = fma(-rql, xl, v);
+                            gq = fma(fabs(rql), fabs(xl), gq);
+                        }
                         const double d = Rq[q + (size_t)q * WB_SM];
-                        Vl[q * WB_SM + j] = (fabs(d) > thr) ? v / d : 0.0;
+                        const bool ok = fabs(d) > 1e-300;
+                        const double x = ok ? v / d : 0.0;
+                        if (!ok) bad = 1;
+                        gmax = fmax(gmax, fma(fabs(d), fabs(x), gq));
+                        Vl[q * WB_SM + j] = x;
                     }
+                    if (!(sg[WB_SM + j] * s * 2.3e-16 * gmax <= tol)) bad = 1;
                 }
             }
-            __syncthreads();
+            if (__syncthreads_or(bad)) {
+                if (tid == 0) tasks_s[e] = RC_REDO;
+                __syncthreads();
+                continue;
+            }
             // ---- outputs as in k_rc_wave: V' to Out + nr NB, then U' in place
             //      over C (each wave reads and writes only its own row tiles) ----
             wv_tall_mm<WB_SM>(NB, s, nr, 1.0,
@@ -1195,6 +1236,7 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, double 
         }
         if (tid == 0) {
             *rank_at(t, m, n) = nr;
+            rank_hist(o, NT, m, n, nr);
             unsigned long long mm = tdim(t, m), ss = s;
             unsigned long long qa = opc('q', mm, ss, 0, 0);
             unsigned long long qb = opc('m', ra, rb, mm, 0) + opc('m', ra, mm, rb, 0) + opc('q', mm, ss, 0, 0);

@@ -1,100 +1,8 @@
-// tlr_kernels.hpp -- HIP kernels of the TLR Cholesky path on gfx950.
-//
-// Device layout (one descriptor, everything resident in HBM):
-//   band tile (m,n), m-n < band : NB x NB, ld NB, at band + ((m-n)*NT + m)*NB*NB
-//   off-band tile (m,n), m > n   : slot of NB*2*cap doubles at lr + (m(m-1)/2+n)*NB*2*cap,
-//                                  compact [U | V]: U = NB x r, V = NB x r at offset NB*r
-//   rank map                     : int32 NT x NT column major, rank[n*NT + m]
-// The matrix is padded to NT*NB with an identity block, so every tile is
-// NB x NB and the factor of the padding is the identity (log 1 = 0).
+// tlr_kernels.hpp -- HIP kernels of the TLR Cholesky path on gfx950
+// (generation, dense and low-rank potrf / trsm / syrk / gemm, utilities).
 #pragma once
 #include "blk.hpp"
-
-struct TLR {
-    double *band;
-    double *lr;
-    int *rank;
-    const double *xyz;
-    int N, NB, NT, band_size, cap, kind, ndim;
-    double sigma2, beta, nugget;
-};
-
-__device__ __forceinline__ double *band_tile(const TLR &t, int m, int n)
-{
-    return t.band + ((size_t)(m - n) * t.NT + m) * (size_t)t.NB * t.NB;
-}
-__device__ __forceinline__ double *lr_tile(const TLR &t, int m, int n)
-{
-    return t.lr + ((size_t)m * (m - 1) / 2 + n) * (size_t)t.NB * 2 * t.cap;
-}
-__device__ __forceinline__ int *rank_at(const TLR &t, int m, int n)
-{
-    return t.rank + (size_t)n * t.NT + m;
-}
-__device__ __forceinline__ int tdim(const TLR &t, int m) { return m == t.NT - 1 ? t.N - m * t.NB : t.NB; }
-
-// covariance entry for global indices (identity on the padding)
-__device__ __forceinline__ double kval(const TLR &t, int gi, int gj)
-{
-    if (gi >= t.N || gj >= t.N) return gi == gj ? 1.0 : 0.0;
-    double d2 = 0.0;
-    for (int d = 0; d < t.ndim; d++) {
-        double df = t.xyz[(size_t)d * t.N + gi] - t.xyz[(size_t)d * t.N + gj];
-        d2 += df * df;
-    }
-    double v = (t.kind == 4) ? t.sigma2 * exp(-sqrt(d2) / t.beta)
-                             : t.sigma2 * exp(-d2 / (2.0 * t.beta * t.beta));
-    if (gi == gj) v += t.nugget;
-    return v;
-}
-
-// ---- operation counts: the reference's op_counts formulas (op_counts.h) ----
-__device__ __forceinline__ unsigned long long opc(char op, unsigned long long a, unsigned long long b,
-                                                  unsigned long long c, unsigned long long d)
-{
-    unsigned long long r = 0;
-    if (op == 'q') {
-        r = 2 * a * b * b - (unsigned long long)((float)(2 * b * b * b) / 3.0f) + 2 * a * b +
-            (unsigned long long)((float)(17 * b) / 3.0f);
-    } else if (op == 'c') {
-        r = (unsigned long long)((double)(a * a * a / 3) - (double)(a * a) / 2.0 + (double)(a / 6));
-    } else if (op == 't') {
-        r = (c == 1) ? b * a * a : a * b * b;
-    } else if (op == 'm') {
-        r = a * b * c * 2;
-    } else if (op == 's') {
-        r = 22 * a * a * a;
-    } else if (op == 'o') {
-        if (d == 1) r = 4 * b * a * c - 2 * b * c * c + 3 * b * c;
-        else r = 4 * b * a * c - 2 * a * c * c + 2 * a * c + b * c - c * c / 2 + c / 2;
-    } else if (op == 'r') {
-        r = (c == 1) ? a * a * b : a * b * b;
-    }
-    return r;
-}
-
-struct OpLog {
-    unsigned long long *tile;  // NT*NT, index m*NT+n
-    unsigned long long *stat;  // band, offband, path, offpath
-};
-
-__device__ __forceinline__ void oplog(const OpLog &o, int NT, unsigned long long cnt, int i, int j,
-                                      bool band, bool path)
-{
-    atomicAdd(o.tile + (size_t)i * NT + j, cnt);
-    atomicAdd(o.stat + (band ? 0 : 1), cnt);
-    atomicAdd(o.stat + (path ? 2 : 3), cnt);
-}
-
-// decode a linear index into (hi, lo) with 0 <= lo < hi
-__device__ __forceinline__ void tri_decode(long long t, int &hi, int &lo)
-{
-    int h = (int)((1.0 + sqrt(1.0 + 8.0 * (double)t)) * 0.5);
-    while ((long long)h * (h - 1) / 2 > t) h--;
-    while ((long long)(h + 1) * h / 2 <= t) h++;
-    hi = h;
-    lo = (int)(t - (long long)h * (h - 1) / 2);
-}
+#include "tlr.hpp"
 
 // ======================= generation =======================
 
@@ -170,7 +78,8 @@ __global__ void __launch_bounds__(BS) k_compress(TLR t, const double *Om, double
         for (int j = 0; j < p; j++)
             for (int i = tid; i < p; i += BS) Mj[i + (size_t)j * p] = (i <= j) ? X[i + (size_t)j * NB] : 0.0;
         __syncthreads();
-        blk_jacobi(p, Mj, p, Vj, p, sig, flag, sm + BLK_GEMM_SMEM);
+        if (blk_jacobi(p, Mj, p, Vj, p, sig, flag, sm + BLK_GEMM_SMEM) >= BLK_JACOBI_SWEEPS && tid == 0)
+            atomicAdd(info + 2, 1);
         blk_sort_pos(p, sig, pos);
         for (int j = tid; j < p; j += BS) ssig[pos[j]] = sig[j];
         __syncthreads();
@@ -233,75 +142,14 @@ __global__ void __launch_bounds__(BS) k_potrf_update(TLR t, int k, int j0, int j
     gemm_tile<false, true>(ti * GT, tj * GT, R, R, jb, -1.0, P, NB, P, NB, 1.0, S, NB, sm, true);
 }
 
-// ---- LDS-resident diagonal block: Cholesky + inverse ----
-// Factors the jb x jb block at A(j0,j0) of tile (k,k) in LDS, writes L back
-// and W = L^{-1} (ld PBK) to Winv[j0/PBK].  One workgroup of BS threads.
 #define PBK 64
 #define PLD (PBK + 1)
-__global__ void __launch_bounds__(BS) k_potrf_diag2(TLR t, int k, int j0, int jb, double *Winv, int *info, OpLog o)
-{
-    // dpotrf of the whole tile counts once, on the band and on the critical path
-    if (j0 == 0 && threadIdx.x == 0) oplog(o, t.NT, opc('c', tdim(t, k), 0, 0, 0), k, k, true, true);
-    __shared__ double L[PBK * PLD];
-    __shared__ double W[PBK * PLD];
-    __shared__ int bad;
-    const int NB = t.NB, tid = threadIdx.x;
-    double *A = band_tile(t, k, k) + j0 + (size_t)j0 * NB;
-    for (int e = tid; e < jb * jb; e += BS) {
-        int i = e % jb, c = e / jb;
-        L[i + c * PLD] = (i >= c) ? A[i + (size_t)c * NB] : 0.0;
-    }
-    if (tid == 0) bad = 0;
-    __syncthreads();
-    for (int j = 0; j < jb; j++) {
-        const double d2 = L[j + j * PLD];
-        if (!(d2 > 0.0)) {
-            if (tid == 0) { atomicCAS(info, 0, k * NB + j0 + j + 1); bad = 1; }
-            break;
-        }
-        const double d = sqrt(d2), rd = 1.0 / d;
-        __syncthreads();
-        if (tid == 0) L[j + j * PLD] = d;
-        for (int i = j + 1 + tid; i < jb; i += BS) L[i + j * PLD] *= rd;
-        __syncthreads();
-        const int r = jb - j - 1;
-        for (int e = tid; e < r * r; e += BS) {
-            int l = j + 1 + e % r, c = j + 1 + e / r;
-            if (l >= c) L[l + c * PLD] -= L[l + j * PLD] * L[c + j * PLD];
-        }
-        __syncthreads();
-    }
-    __syncthreads();
-    if (bad) return;
-    // W = L^{-1}: 4 adjacent lanes per column, rows in sequence; lane q keeps
-    // the rows i with i % 4 == q of its column (same-lane LDS reuse)
-    {
-        const int c = tid >> 2, q = tid & 3;
-        const bool act = c < jb;
-        for (int i = 0; i < jb; i++) {
-            double s = 0.0;
-            if (act && i > c)
-                for (int l = c + ((q - c) & 3); l < i; l += 4) s = fma(L[i + l * PLD], W[l + c * PLD], s);
-            s = gsum<4>(s);
-            if (act && (i & 3) == q) {
-                double w = (i < c) ? 0.0 : (i == c ? 1.0 / L[c + c * PLD] : -s / L[i + i * PLD]);
-                W[i + c * PLD] = w;
-            }
-        }
-    }
-    __syncthreads();
-    double *Wo = Winv + (size_t)(j0 / PBK) * PBK * PBK;
-    for (int e = tid; e < jb * jb; e += BS) {
-        int i = e % jb, c = e / jb;
-        if (i >= c) A[i + (size_t)c * NB] = L[i + c * PLD];
-        Wo[i + c * PBK] = (i >= c) ? W[i + c * PLD] : 0.0;
-    }
-}
 
-// Same contract as k_potrf_diag2, one wave: left-looking Cholesky with the
-// block row-major in LDS (lane i owns row i; the pivot row is read as an LDS
-// broadcast and the pivot itself by readlane), then W = L^{-1} with lane c
-// owning column c (forward substitution, L rows read as broadcasts).
+// Factors the jb x jb diagonal block at A(j0, j0) of tile (k, k) with one
+// wave, writes L back and W = L^{-1} (ld PBK) to Winv[j0 / PBK]: right-looking
+// Cholesky with lane i owning row i in registers (pivots by readlane, each
+// column of L broadcast through LDS), then W column by column with lane c
+// owning column c.
 #define PLR (PBK + 2)
 __global__ void __launch_bounds__(64) k_potrf_diag3(TLR t, int k, int j0, int jb, double *Winv, int *info, OpLog o)
 {
@@ -419,62 +267,6 @@ __global__ void __launch_bounds__(BS) k_potrf_panel2(TLR t, int k, int j0, int j
     for (int q = 0; q < PBK / (BS / PBK); q++) {
         const int c = cg + q * (BS / PBK);
         if (c < jb && i < nr) A[r0 + i + (size_t)(j0 + c) * NB] = x[q];
-    }
-}
-
-// low-rank tiles m = k+band .. NT-1: V <- L^{-1} V by left-looking block
-// substitution with the inverted diagonal blocks of L; grid (ceil(cap/16), count)
-__global__ void __launch_bounds__(BS) k_trsm_lr2(TLR t, int k, const double *Winv, OpLog o)
-{
-    __shared__ double Vb[PBK * 17];
-    const int NB = t.NB, tid = threadIdx.x;
-    const int m = k + t.band_size + blockIdx.y;
-    const int r = *rank_at(t, m, k);
-    if (blockIdx.x == 0 && tid == 0) oplog(o, t.NT, opc('t', tdim(t, m), r, 1, 0), m, k, false, m - k == 1);
-    const int c0 = blockIdx.x * 16;
-    if (c0 >= r) return;
-    const int nc = (r - c0) < 16 ? (r - c0) : 16;
-    double *V = lr_tile(t, m, k) + (size_t)NB * r + (size_t)c0 * NB;
-    const double *L = band_tile(t, k, k);
-    const int i = tid % PBK, cg = tid / PBK;  // 4 column groups of 4
-    for (int b0 = 0; b0 < NB; b0 += PBK) {
-        const int nb = (NB - b0) < PBK ? (NB - b0) : PBK;
-        double acc[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int c = cg * 4 + q;
-            acc[q] = (i < nb && c < nc) ? V[b0 + i + (size_t)c * NB] : 0.0;
-        }
-        if (i < nb) {
-            const double *Lr = L + b0 + i;
-#pragma unroll 4
-            for (int l = 0; l < b0; l++) {
-                const double lv = Lr[(size_t)l * NB];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int c = cg * 4 + q;
-                    if (c < nc) acc[q] = fma(-lv, V[l + (size_t)c * NB], acc[q]);
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++) Vb[i + (cg * 4 + q) * PBK] = acc[q];
-        __syncthreads();
-        const double *Wb = Winv + (size_t)(b0 / PBK) * PBK * PBK;
-        double x[4] = {0.0, 0.0, 0.0, 0.0};
-        if (i < nb) {
-            for (int l = 0; l <= i; l++) {
-                const double w = Wb[i + (size_t)l * PBK];
-#pragma unroll
-                for (int q = 0; q < 4; q++) x[q] = fma(w, Vb[l + (cg * 4 + q) * PBK], x[q]);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int c = cg * 4 + q;
-            if (i < nb && c < nc) V[b0 + i + (size_t)c * NB] = x[q];
-        }
-        __syncthreads();
     }
 }
 
@@ -721,6 +513,19 @@ __global__ void __launch_bounds__(BS) k_gemm_lrd_update(TLR t, int k, const doub
 
 // ======================= gemm, low-rank result (cases iv, v) =======================
 
+// tasks_s value of a task handed from a fast recompression kernel to the
+// Householder path (k_lr_gemm, launched after them on the context stream)
+#define RC_REDO (1 << 30)
+
+// Accepted error of the V' = Q (Rq^-1 Vs_r) back-substitution of the fast
+// kernels, absolute in the tile: a fraction of the truncation threshold,
+// never below rounding of M (fro = ||M||_F).
+__device__ __forceinline__ double rc_solve_tol(double acc, int fixed_rk, double fro)
+{
+    const double floor_ = 1e-13 * fro;
+    return (fixed_rk > 0 || !(acc > 0.0)) ? floor_ : fmax(1e-3 * acc, floor_);
+}
+
 struct RcWs {
     size_t stride;  // doubles per slot
     int smax;       // 2*cap
@@ -808,7 +613,8 @@ __global__ void __launch_bounds__(BS) k_lr_gemm(TLR t, int k, const int *tasks_s
                 M[e] = v;
             }
             __syncthreads();
-            blk_jacobi(s, M, s, Vs, s, sig, flag, sm + BLK_GEMM_SMEM);
+            if (blk_jacobi(s, M, s, Vs, s, sig, flag, sm + BLK_GEMM_SMEM) >= BLK_JACOBI_SWEEPS && tid == 0)
+                atomicAdd(o.unconv, 1);
             blk_sort_pos(s, sig, pos);
             if (tid == 0) {
                 int c = 0;
@@ -835,6 +641,7 @@ __global__ void __launch_bounds__(BS) k_lr_gemm(TLR t, int k, const int *tasks_s
         }
         if (tid == 0) {
             *rank_at(t, m, n) = nr;
+            rank_hist(o, NT, m, n, nr);
             unsigned long long nn = tdim(t, m), ss = s;  // the reference counts with tile m's rows
             unsigned long long qa = opc('q', nn, ss, 0, 0), qb;
             if (bdense) qb = opc('q', nn, ss, 0, 0) + opc('m', ra, nn, nn, 0);
@@ -866,6 +673,24 @@ __global__ void __launch_bounds__(BS) k_logdet(TLR t, double *out)
     }
     double s = blk_sum(part, red);
     if (threadIdx.x == 0) *out = 2.0 * s;
+}
+
+// rank history of the off-band tiles: phase 0 sets initial = min = max =
+// final = the current rank, phase 1 the final rank
+// (hicma_parsec/HiCMA_dpotrf_L_2flow.jdf:353-361, 1100-1110); 0 elsewhere
+__global__ void k_rank_hist(TLR t, int *h, int phase)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= t.NT * t.NT) return;
+    int m = e % t.NT, n = e / t.NT;
+    int *p = h + 4 * (size_t)e;
+    int r = (m - n >= t.band_size) ? t.rank[e] : 0;
+    if (phase == 0) {
+        p[0] = r;
+        p[1] = r;
+        p[2] = r;
+    }
+    p[3] = r;
 }
 
 // rank map entries of band tiles set to -1 (rank_check.jdf on_band)

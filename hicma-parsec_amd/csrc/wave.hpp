@@ -115,3 +115,12 @@ __device__ __forceinline__ double wrs16(double (&p)[16])
     v += dppd<DPP_XOR1>(v);
     return v;
 }
+
+// LDS ordering inside one wave: a code-motion barrier plus wavefront-scope
+// fences (LDS operations of one wave complete in order)
+__device__ __forceinline__ void wv_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}

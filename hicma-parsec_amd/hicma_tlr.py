@@ -2,7 +2,8 @@
 
 Mirrors the reference's C interface (hicma_parsec/hicma_parsec.h:38-211):
 ``STARSH_gen``, ``parsec_rank_gather``, ``parsec_band_size_auto_tuning``,
-``parsec_band_regenerate``, ``HiCMA_dpotrf_L``, ``STARSH_check``.  The binding
+``parsec_band_regenerate``, ``HiCMA_dpotrf_L``, ``STARSH_check``, and the
+harness's checks (``check_dpotrf2``, ``check_diff``, hicma_parsec/testing_dpotrf.c:111-230).  The binding
 only marshals arguments; every computation runs in the HIP library, and
 loading fails loudly if the library is missing.
 """
@@ -26,10 +27,33 @@ EXPORTS = [
     "HICMA_get_stat2", "parsec_band_size_auto_tuning", "hicma_band_autotune", "parsec_band_regenerate",
     "HiCMA_dpotrf_L", "hicma_op_counts", "hicma_op_stats", "hicma_rank_overflows", "STARSH_check",
     "hicma_get_tile", "hicma_set_tile", "hicma_get_rank_map", "hicma_logdet", "hicma_recompress_stats",
-    "hicma_profile", "hicma_profile_read", "hicma_desc_copy",
+    "hicma_profile", "hicma_profile_read", "hicma_desc_copy", "hicma_jacobi_unconverged", "hicma_exec_op_stats",
+    "hicma_rank_history", "hicma_get_rank_history", "parsec_rank_print", "hicma_dense_create", "hicma_dense_destroy", "hicma_dense_get",
+    "hicma_dense_set", "hicma_dense_generate", "hicma_dense_potrf", "hicma_dense_lansy", "hicma_dense_diff_lansy",
+    "hicma_dense_logdet", "hicma_check_dpotrf2", "hicma_check_diff", "hicma_tlr_trsm", "hicma_tlr_potrs",
+    "hicma_tlr_loglik", "hicma_dev_malloc", "hicma_dev_free", "hicma_memcpy",
+    "hicma_init_ex", "hicma_server_create", "hicma_server_submit", "hicma_server_wait", "hicma_server_destroy",
+    "hicma_normal",
 ]
 
 _lib = None
+
+
+class LikRequest(C.Structure):
+    """hicma_lik_request"""
+    _fields_ = [("N", C.c_int), ("NB", C.c_int), ("maxrank", C.c_int), ("eps", C.c_double),
+                ("sigma2", C.c_double), ("beta", C.c_double), ("nugget", C.c_double), ("seed", C.c_uint64),
+                ("z", C.c_void_p), ("zseed", C.c_uint64)]
+
+
+class LikResult(C.Structure):
+    """hicma_lik_result"""
+    _fields_ = [("status", C.c_int), ("band", C.c_int), ("logdet", C.c_double), ("quad", C.c_double),
+                ("loglik", C.c_double), ("t_queue", C.c_double), ("t_run", C.c_double), ("t_total", C.c_double),
+                ("bytes", C.c_size_t)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 def lib():
@@ -47,19 +71,19 @@ def lib():
             "hicma_desc_info": (i, [vp, ip, ip, ip, ip, ip]),
             "hicma_get_points": (i, [vp, dp]),
             "hicma_points": (i, [i, i, u64, dp]),
-            "STARSH_gen": (i, [vp, i, vp, vp, d, i, i, i, C.POINTER(i)]),
+            "STARSH_gen": (i, [vp, i, vp, vp, vp, vp, vp, d, i, i, i, C.POINTER(i)]),
             "parsec_rank_gather": (i, [vp, vp, ip, i]),
             "parsec_rank_check": (i, [vp, vp, i]),
             "HICMA_get_stat2": (None, [ip, i, i, C.POINTER(i), C.POINTER(i), C.POINTER(d)]),
             "parsec_band_size_auto_tuning": (i, [vp, vp, vp, ip, i]),
             "hicma_band_autotune": (i, [ip, i, i]),
-            "parsec_band_regenerate": (i, [vp, i, vp, i]),
+            "parsec_band_regenerate": (i, [vp, i, vp, vp, vp, vp, i]),
             "HiCMA_dpotrf_L": (i, [vp, i, vp, vp, vp, d, i, i, C.POINTER(i), i, i, i, i, C.POINTER(i),
                                    vp, vp, vp, i]),
             "hicma_op_counts": (C.c_ulong, [C.c_char, C.c_ulong, C.c_ulong, C.c_ulong, C.c_ulong]),
             "hicma_op_stats": (i, [vp, vp]),
             "hicma_rank_overflows": (i, [vp]),
-            "STARSH_check": (i, [vp, i, dp, vp, vp, i, i, C.POINTER(i)]),
+            "STARSH_check": (i, [vp, i, vp, vp, vp, i, i, C.POINTER(i)]),
             "hicma_get_tile": (i, [vp, i, i, dp, C.POINTER(i)]),
             "hicma_set_tile": (i, [vp, i, i, dp, i]),
             "hicma_get_rank_map": (i, [vp, ip]),
@@ -68,6 +92,34 @@ def lib():
             "hicma_profile": (i, [vp, i]),
             "hicma_profile_read": (i, [vp, C.POINTER(d), C.POINTER(i)]),
             "hicma_desc_copy": (i, [vp, vp]),
+            "hicma_jacobi_unconverged": (i, [vp]),
+            "hicma_exec_op_stats": (i, [vp, vp, i]),
+            "hicma_rank_history": (i, [vp, i]),
+            "hicma_get_rank_history": (i, [vp, vp]),
+            "parsec_rank_print": (i, [vp, vp, i]),
+            "hicma_dense_create": (i, [vp, C.POINTER(vp), i]),
+            "hicma_dense_destroy": (i, [vp]),
+            "hicma_dense_get": (i, [vp, dp]),
+            "hicma_dense_set": (i, [vp, dp]),
+            "hicma_dense_generate": (i, [vp, i, vp, vp]),
+            "hicma_dense_potrf": (i, [vp, i, vp, C.POINTER(i)]),
+            "hicma_dense_lansy": (i, [vp, i, vp, C.POINTER(d)]),
+            "hicma_dense_diff_lansy": (i, [vp, i, vp, vp, C.POINTER(d)]),
+            "hicma_dense_logdet": (i, [vp, vp, C.POINTER(d)]),
+            "hicma_check_dpotrf2": (i, [vp, i, i, vp, vp, d, C.POINTER(d)]),
+            "hicma_check_diff": (i, [vp, i, i, vp, vp, d, C.POINTER(d)]),
+            "hicma_tlr_trsm": (i, [vp, vp, i, i, vp, i]),
+            "hicma_tlr_potrs": (i, [vp, vp, i, vp, i]),
+            "hicma_tlr_loglik": (i, [vp, vp, vp, C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
+            "hicma_dev_malloc": (i, [vp, C.c_size_t, C.POINTER(vp)]),
+            "hicma_dev_free": (i, [vp, vp]),
+            "hicma_memcpy": (i, [vp, vp, vp, C.c_size_t, i]),
+            "hicma_init_ex": (i, [C.POINTER(vp), i, i, i]),
+            "hicma_server_create": (i, [C.POINTER(vp), i, i, d]),
+            "hicma_server_submit": (i, [vp, C.POINTER(LikRequest), C.POINTER(C.c_uint64)]),
+            "hicma_server_wait": (i, [vp, C.c_uint64, C.POINTER(LikResult)]),
+            "hicma_server_destroy": (i, [vp]),
+            "hicma_normal": (i, [u64, i, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -105,6 +157,43 @@ def rank_stat(G, NT, band):
     return mn.value, mx.value, av.value
 
 
+def normal(seed, n):
+    z = np.zeros(n)
+    _check(lib().hicma_normal(seed, n, z.ctypes.data), "hicma_normal")
+    return z
+
+
+class Server:
+    """Continuous batching of Gaussian log-likelihood requests (hicma_server_*)."""
+
+    def __init__(self, device=0, slots=4, hbm_budget_gb=200.0):
+        self.h = C.c_void_p()
+        _check(lib().hicma_server_create(C.byref(self.h), device, slots, hbm_budget_gb), "hicma_server_create")
+
+    def submit(self, N, sigma2, beta, nugget=1e-2, seed=0, z=None, zseed=0, NB=512, maxrank=0, eps=1e-8):
+        zz = None if z is None else np.ascontiguousarray(z, dtype=np.float64)
+        r = LikRequest(N, NB, maxrank, eps, sigma2, beta, nugget, seed, None if zz is None else zz.ctypes.data, zseed)
+        t = C.c_uint64()
+        _check(lib().hicma_server_submit(self.h, C.byref(r), C.byref(t)), "hicma_server_submit")
+        return t.value
+
+    def wait(self, ticket):
+        res = LikResult()
+        _check(lib().hicma_server_wait(self.h, ticket, C.byref(res)), "hicma_server_wait")
+        return res.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().hicma_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 class Context:
     def __init__(self, device=0):
         self.h = C.c_void_p()
@@ -128,6 +217,119 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+
+class DeviceArray:
+    """A float64 array in HBM (N x k column major), for the solves."""
+
+    def __init__(self, ctx, host):
+        self.ctx = ctx
+        self.host = np.array(host, dtype=np.float64, order="F", copy=True)
+        self.p = C.c_void_p()
+        _check(lib().hicma_dev_malloc(ctx.h, self.host.nbytes, C.byref(self.p)), "hicma_dev_malloc")
+        _check(lib().hicma_memcpy(ctx.h, self.p, self.host.ctypes.data, self.host.nbytes, 1), "hicma_memcpy")
+
+    def get(self):
+        out = np.empty_like(self.host, order="F")
+        _check(lib().hicma_memcpy(self.ctx.h, out.ctypes.data, self.p, out.nbytes, 0), "hicma_memcpy")
+        return out
+
+    def close(self):
+        if self.p:
+            lib().hicma_dev_free(self.ctx.h, self.p)
+            self.p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class KernelParams(C.Structure):
+    """hicma_kernel_params: the `data` argument of STARSH_gen."""
+    _fields_ = [("sigma2", C.c_double), ("beta", C.c_double), ("nugget", C.c_double)]
+
+
+def _problem_args(params, kernel, index, N):
+    data = C.byref(KernelParams(*params)) if params is not None else None
+    kern = C.byref(C.c_int(kernel)) if kernel is not None else None
+    idx = None
+    if index is not None:
+        idx = np.ascontiguousarray(index, dtype=np.int64)
+        assert idx.shape == (N,)
+    return data, kern, idx
+
+
+class Dense:
+    """An N x N column-major matrix in HBM (the --check's dense descriptors)."""
+
+    def __init__(self, ctx, N):
+        self.ctx, self.N = ctx, N
+        self.h = C.c_void_p()
+        _check(lib().hicma_dense_create(ctx.h, C.byref(self.h), N), "hicma_dense_create")
+
+    def close(self):
+        if self.h:
+            lib().hicma_dense_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get(self):
+        D = np.zeros((self.N, self.N), order="F")
+        _check(lib().hicma_dense_get(self.h, D.ctypes.data), "hicma_dense_get")
+        return D
+
+    def set(self, D):
+        D = np.asfortranarray(D, dtype=np.float64)
+        _check(lib().hicma_dense_set(self.h, D.ctypes.data), "hicma_dense_set")
+
+    def generate(self, problem):
+        _check(lib().hicma_dense_generate(self.ctx.h, LOWER, self.h, problem.h), "hicma_dense_generate")
+
+    def uncompress(self, A):
+        info = C.c_int(0)
+        _check(lib().STARSH_check(self.ctx.h, LOWER, self.h, A.h, A.h, A.band, A.maxrank, C.byref(info)),
+               "STARSH_check")
+
+    def potrf(self):
+        info = C.c_int(0)
+        _check(lib().hicma_dense_potrf(self.ctx.h, LOWER, self.h, C.byref(info)), "hicma_dense_potrf")
+        return info.value
+
+    def lansy(self, other=None):
+        v = C.c_double()
+        if other is None:
+            _check(lib().hicma_dense_lansy(self.ctx.h, LOWER, self.h, C.byref(v)), "hicma_dense_lansy")
+        else:
+            _check(lib().hicma_dense_diff_lansy(self.ctx.h, LOWER, self.h, other.h, C.byref(v)),
+                   "hicma_dense_diff_lansy")
+        return v.value
+
+    def logdet(self):
+        v = C.c_double()
+        _check(lib().hicma_dense_logdet(self.ctx.h, self.h, C.byref(v)), "hicma_dense_logdet")
+        return v.value
+
+
+def check_dpotrf2(ctx, L, A0, threshold, verbose=0):
+    """(info, ||A0 - L L^T|| / ||A0||) on the device."""
+    r = C.c_double()
+    bad = _check(lib().hicma_check_dpotrf2(ctx.h, verbose, LOWER, L.h, A0.h, threshold, C.byref(r)),
+                 "hicma_check_dpotrf2")
+    return bad, r.value
+
+
+def check_diff(ctx, L, L0, threshold, verbose=0):
+    """(info, ||L - L0|| / ||L||) on the device."""
+    r = C.c_double()
+    bad = _check(lib().hicma_check_diff(ctx.h, verbose, LOWER, L.h, L0.h, threshold, C.byref(r)), "hicma_check_diff")
+    return bad, r.value
 
 
 class TLRMatrix:
@@ -162,10 +364,14 @@ class TLRMatrix:
         _check(lib().hicma_get_points(self.h, xyz.ctypes.data), "hicma_get_points")
         return xyz.reshape(self.ndim, self.N)
 
-    def generate(self, tol=1e-8, maxrank=None, band_size=1):
+    def generate(self, tol=1e-8, maxrank=None, band_size=1, params=None, kernel=None, index=None):
+        """STARSH_gen; params = (sigma2, beta, nugget), kernel = a kind,
+        index = point index of each matrix row (None: keep the descriptor's)."""
         info = C.c_int(0)
-        _check(lib().STARSH_gen(self.ctx.h, LOWER, self.h, self.h, tol, maxrank or self.maxrank, band_size, 0,
-                                C.byref(info)), "STARSH_gen")
+        data, kern, idx = _problem_args(params, kernel, index, self.N)
+        _check(lib().STARSH_gen(self.ctx.h, LOWER, self.h, self.h, data, kern,
+                                idx.ctypes.data if idx is not None else None, tol, maxrank or self.maxrank,
+                                band_size, 0, C.byref(info)), "STARSH_gen")
         self.band = band_size
         return info.value
 
@@ -185,27 +391,78 @@ class TLRMatrix:
         return _check(lib().parsec_band_size_auto_tuning(self.ctx.h, self.h, None, G.ctypes.data, self.NB),
                       "parsec_band_size_auto_tuning")
 
-    def regenerate(self, band_size):
-        _check(lib().parsec_band_regenerate(self.ctx.h, LOWER, self.h, band_size), "parsec_band_regenerate")
+    def regenerate(self, band_size, params=None, kernel=None, index=None):
+        data, kern, idx = _problem_args(params, kernel, index, self.N)
+        _check(lib().parsec_band_regenerate(self.ctx.h, LOWER, self.h, data, kern,
+                                            idx.ctypes.data if idx is not None else None, band_size),
+               "parsec_band_regenerate")
         self.band = band_size
 
-    def potrf(self, acc=1e-8, fixed_rank=0, maxrank=None, lookahead=-1, two_flow=0, want_ops=False):
+    def potrf(self, acc=1e-8, fixed_rank=0, maxrank=None, lookahead=-1, two_flow=0, want_ops=False,
+              critical_path=False):
         la, tf = C.c_int(lookahead), C.c_int(two_flow)
         tile = np.zeros(self.NT * self.NT, dtype=np.uint64) if want_ops else None
         opc = np.zeros(4, dtype=np.uint64)
-        cpt = np.zeros(6)
+        cpt = np.zeros(6) if critical_path else None
         mr = maxrank or self.maxrank
         info = lib().HiCMA_dpotrf_L(self.ctx.h, LOWER, self.h, self.h, self.h, acc, fixed_rank, mr, C.byref(la),
                                     self.band, 300, mr, 0, C.byref(tf),
-                                    tile.ctypes.data if want_ops else None, opc.ctypes.data, cpt.ctypes.data, 0)
+                                    tile.ctypes.data if want_ops else None, opc.ctypes.data,
+                                    cpt.ctypes.data if critical_path else None, 0)
         _check(info, "HiCMA_dpotrf_L")
         st = np.zeros(4, dtype=np.uint64)
         lib().hicma_op_stats(self.h, st.ctypes.data)
         self.stats = st
+        ex = np.zeros(4, dtype=np.uint64)
+        lib().hicma_exec_op_stats(self.h, ex.ctypes.data, 4)
+        self.exec_stats = ex
         self.two_flow = tf.value
         self.critical_path_time = cpt
         self.tileops = tile.reshape(self.NT, self.NT) if want_ops else None
         return info
+
+    def trsm(self, B, trans=0):
+        """L^-1 B (trans 0) or L^-T B (trans 1) on the device; B host array N x k or N."""
+        X = DeviceArray(self.ctx, np.reshape(B, (self.N, -1), order="F"))
+        _check(lib().hicma_tlr_trsm(self.ctx.h, self.h, trans, X.host.shape[1], X.p, self.N), "hicma_tlr_trsm")
+        out = X.get()
+        X.close()
+        return out.reshape(np.shape(B), order="F")
+
+    def potrs(self, B):
+        X = DeviceArray(self.ctx, np.reshape(B, (self.N, -1), order="F"))
+        _check(lib().hicma_tlr_potrs(self.ctx.h, self.h, X.host.shape[1], X.p, self.N), "hicma_tlr_potrs")
+        out = X.get()
+        X.close()
+        return out.reshape(np.shape(B), order="F")
+
+    def loglik(self, z):
+        """(logdet, z^T Sigma^-1 z, log-likelihood) of the host vector z."""
+        X = DeviceArray(self.ctx, np.asarray(z, dtype=np.float64).reshape(self.N, 1))
+        ld, qd, ll = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().hicma_tlr_loglik(self.ctx.h, self.h, X.p, C.byref(ld), C.byref(qd), C.byref(ll)),
+               "hicma_tlr_loglik")
+        X.close()
+        return ld.value, qd.value, ll.value
+
+    def jacobi_unconverged(self):
+        return lib().hicma_jacobi_unconverged(self.h)
+
+    def rank_history(self, enable=True):
+        _check(lib().hicma_rank_history(self.h, int(enable)), "hicma_rank_history")
+
+    def get_rank_history(self):
+        """[m, n, q] with q = initial, min, max, final rank of the last factorization."""
+        h = np.zeros(4 * self.NT * self.NT, dtype=np.int32)
+        _check(lib().hicma_get_rank_history(self.h, h.ctypes.data), "hicma_get_rank_history")
+        return h.reshape(self.NT, self.NT, 4).transpose(1, 0, 2)
+
+    def rank_print(self):
+        _check(lib().parsec_rank_print(self.ctx.h, self.h, self.band), "parsec_rank_print")
+
+    def rank_check(self):
+        """parsec_rank_check: number of negative off-band ranks; band entries set to -1."""
+        return _check(lib().parsec_rank_check(self.ctx.h, self.h, self.band), "parsec_rank_check")
 
     def recompress_stats(self):
         st = np.zeros(3, dtype=np.uint64)
@@ -225,11 +482,14 @@ class TLRMatrix:
         return v.value
 
     def uncompress(self):
-        D = np.zeros((self.N, self.N), order="F")
-        info = C.c_int(0)
-        _check(lib().STARSH_check(self.ctx.h, LOWER, D.ctypes.data, self.h, self.h, self.band, self.maxrank,
-                                  C.byref(info)), "STARSH_check")
-        return D
+        """Host copy of the uncompressed lower triangle (STARSH_check into a
+        device matrix, then copied back)."""
+        D = Dense(self.ctx, self.N)
+        try:
+            D.uncompress(self)
+            return D.get()
+        finally:
+            D.close()
 
     def get_tile(self, m, n):
         buf = np.zeros(self.NB * max(self.NB, 2 * self.maxrank))

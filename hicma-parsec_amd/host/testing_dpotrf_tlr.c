@@ -4,8 +4,13 @@
  * (hicma_parsec/testing_dpotrf.c:262-1107, flags hicma_parsec/hicma_parsec.c:92-309).
  *
  * Phases: STARSH_gen -> rank gather -> band_size_auto_tuning ->
- * band_regeneration -> HiCMA_potrf (timed region) -> rank gather -> R-LO record.
- * Added flags (reference defaults): --beta 0.1 --nugget 0.1 --sigma2 1 --seed 0.
+ * band_regeneration -> [check: approximation error, STARSH_check + dense
+ * potrf of the approximate matrix] -> HiCMA_potrf (timed region) -> [check:
+ * check_diff against the dense factor, check_dpotrf2 against the exact
+ * matrix, log-det against the dense log-det] -> rank gather -> R-LO record.
+ * With --check the exit status ORs in both checks, as the reference's ret.
+ * Added flags (reference defaults): --beta 0.1 --nugget 0.1 --sigma2 1 --seed 0;
+ * --print-rank prints the rank maps (the reference's PRINT_RANK build option).
  */
 #define _POSIX_C_SOURCE 200809L
 #include <getopt.h>
@@ -40,7 +45,8 @@ static void usage(void)
             " -Z --band     : band size (default 1)\n"
             " -E --auto-band: auto-tune band size (default 1)\n"
             " -Y --lookahead, -W --two-flow, -F --send_full_tile, -z --HNB, -c --cores, -P, -Q\n"
-            " -x --check    : log-det check\n"
+            " -x --check    : dense checks on the device (check_diff, check_dpotrf2, log-det)\n"
+            " --print-rank  : print the initial/min/max/final rank maps\n"
             " -v --verbose\n"
             " --beta, --nugget, --sigma2, --seed : kernel parameters\n");
 }
@@ -49,7 +55,7 @@ int main(int argc, char **argv)
 {
     int N = 0, NB = 0, fixedrk = 0, maxrank = 0, genmaxrank = 0, compmaxrank = 0, kind = 2;
     int band_size = 1, auto_band = 1, lookahead = -1, two_flow = 0, send_full_tile = 0, HNB = 300;
-    int cores = 1, P = 1, Q = 1, check = 0, verbose = 0;
+    int cores = 1, P = 1, Q = 1, check = 0, verbose = 0, print_rank = 0;
     double tol = 1e-8, beta = 0.1, nugget = 0.1, sigma2 = 1.0;
     unsigned long long seed = 0;
     static struct option lo[] = {
@@ -73,7 +79,8 @@ int main(int argc, char **argv)
         {"E", required_argument, 0, 'E'},        {"two-flow", required_argument, 0, 'W'},
         {"W", required_argument, 0, 'W'},        {"beta", required_argument, 0, 1001},
         {"nugget", required_argument, 0, 1002},  {"sigma2", required_argument, 0, 1003},
-        {"seed", required_argument, 0, 1004},    {0, 0, 0, 0}};
+        {"seed", required_argument, 0, 1004},    {"print-rank", no_argument, 0, 1005},
+        {0, 0, 0, 0}};
     int c, opt = 0;
     while ((c = getopt_long_only(argc, argv, "", lo, &opt)) != -1) {
         switch (c) {
@@ -100,6 +107,7 @@ int main(int argc, char **argv)
         case 1002: nugget = atof(optarg); break;
         case 1003: sigma2 = atof(optarg); break;
         case 1004: seed = strtoull(optarg, NULL, 10); break;
+        case 1005: print_rank = 1; break;
         case 'h': usage(); return 0;
         default: usage(); return 1;
         }
@@ -138,7 +146,7 @@ int main(int argc, char **argv)
 
     int info = 0, ret = 0;
     double t0 = wtime();
-    rc = STARSH_gen(ctx, HICMA_LOWER, A, A, tol, genmaxrank, band_size, send_full_tile, &info);
+    rc = STARSH_gen(ctx, HICMA_LOWER, A, A, NULL, NULL, NULL, tol, genmaxrank, band_size, send_full_tile, &info);
     double time_starsh = wtime() - t0;
     printf("[****] TIME(s) %12.5f : STARSH_gen\tPxQ= %3d %-3d NB= %4d N= %7d\n", time_starsh, P, Q, NB, N);
     if (rc || info != 0) {
@@ -165,7 +173,7 @@ int main(int argc, char **argv)
         t0 = wtime();
         if (opt != 1) {
             band_size = opt;
-            parsec_band_regenerate(ctx, HICMA_LOWER, A, band_size);
+            parsec_band_regenerate(ctx, HICMA_LOWER, A, NULL, NULL, NULL, band_size);
         }
         time_regenerate = wtime() - t0;
         printf("[****] TIME(s) %12.5f : band_regeneration\tPxQ= %3d %-3d NB= %4d N= %7d maxrank= %d "
@@ -186,8 +194,41 @@ int main(int argc, char **argv)
            "send_full= %d memory_per_node= %lf GB memory_per_node_max= %lf GB\n",
            0.0, P, Q, NB, N, maxrank, band_size, send_full_tile, mem, mem_max);
 
+    /* --check before the factorization (hicma_parsec/testing_dpotrf.c:640-771):
+     * approximation error, then the dense factor L0 of the approximate matrix */
+    hicma_dense *dAd = NULL, *dA0 = NULL;
+    if (check) {
+        if (hicma_dense_create(ctx, &dAd, N) || hicma_dense_create(ctx, &dA0, N)) {
+            fprintf(stderr, "not enough device memory for the dense check matrices (2 x %.1f GB)\n",
+                    8.0 * N * (double)N / 1e9);
+            return 1;
+        }
+        hicma_dense_generate(ctx, HICMA_LOWER, dAd, A);
+        t0 = wtime();
+        STARSH_check(ctx, HICMA_LOWER, dA0, A, A, band_size, maxrank, &info);
+        printf("[****] TIME(s) %12.5f : STARSH_check\tPxQ= %3d %-3d NB= %4d N= %7d\n", wtime() - t0, P, Q, NB, N);
+        double norm1 = 0, norm2 = 0, diff = 0;
+        hicma_dense_lansy(ctx, HICMA_LOWER, dAd, &norm1);
+        hicma_dense_lansy(ctx, HICMA_LOWER, dA0, &norm2);
+        printf("Check matrix generation:\nNorms: map=%e lr_jdf=%e\n", norm1, norm2);
+        hicma_dense_diff_lansy(ctx, HICMA_LOWER, dA0, dAd, &diff);
+        printf("Norms: map-lr_jdf=%e\nRelative error of approximation: %e\n\n", diff, diff / norm1);
+        double dflops = (double)N * N * N / 3.0;
+        t0 = wtime();
+        int dinfo = 0;
+        hicma_dense_potrf(ctx, HICMA_LOWER, dA0, &dinfo);
+        double tp = wtime() - t0;
+        printf("[****] TIME(s) %12.5f : Dense_potrf\tPxQ= %3d %-3d NB= %4d N= %7d : %14f gflops\n", tp, P, Q, NB, N,
+               dflops / 1e9 / tp);
+        if (dinfo != 0) {
+            printf("-- dense factorization of the approximate matrix failed (info = %d)\n", dinfo);
+            ret |= 1;
+        }
+    }
+    if (print_rank) hicma_rank_history(A, 1);
+
     unsigned long *tileops = (unsigned long *)calloc((size_t)NT * NT, sizeof(unsigned long));
-    unsigned long opc[1] = {0};
+    unsigned long *opc = (unsigned long *)calloc((size_t)(cores > 4 ? cores : 4), sizeof(unsigned long));
     double cpt[6] = {0};
     double flops = (double)N * ((1. / 6.) * N + 0.5) * N + (1. / 3.) * N + (double)N * ((1. / 6.) * N) * N - N / 6.0;
     t0 = wtime();
@@ -202,6 +243,30 @@ int main(int argc, char **argv)
         printf("-- Factorization is suspicious (info = %d) ! \n", info);
         ret |= 1;
     }
+    if (print_rank) parsec_rank_print(ctx, A, band_size);
+    if (check) {
+        printf("\nCHECK RESULT:\n\n");
+        /* check_diff: L of the TLR factorization against the dense factor of the approximate matrix */
+        STARSH_check(ctx, HICMA_LOWER, dAd, A, A, band_size, maxrank, &info);
+        printf("+++++++++++++++++++++++++++\nChecking HiCMA L vs dense L\n");
+        double r1 = 0, r2 = 0;
+        int bad = hicma_check_diff(ctx, 1, HICMA_LOWER, dAd, dA0, tol, &r1);
+        ret |= (bad != 0);
+        /* check_dpotrf2: L L^T against the exact matrix */
+        printf("++++++++++++++++++++++++++\nChecking HiCMA L vs original A via computing LLT\n");
+        hicma_dense_generate(ctx, HICMA_LOWER, dA0, A);
+        bad = hicma_check_dpotrf2(ctx, 1, HICMA_LOWER, dAd, dA0, tol, &r2);
+        ret |= (bad != 0);
+        /* log-det against the dense Cholesky of the exact matrix */
+        double ld = 0, ldd = 0;
+        int dinfo = 0;
+        hicma_logdet(A, &ld);
+        hicma_dense_potrf(ctx, HICMA_LOWER, dA0, &dinfo);
+        hicma_dense_logdet(ctx, dA0, &ldd);
+        printf("log det = %.15e dense = %.15e relative error = %e\n", ld, ldd, fabs(ld - ldd) / fabs(ldd));
+        hicma_dense_destroy(dAd);
+        hicma_dense_destroy(dA0);
+    }
     int fmin = -1, fmax = -1;
     double favg = -1;
     parsec_rank_gather(ctx, A, G, band_size);
@@ -215,13 +280,17 @@ int main(int argc, char **argv)
            "ratio_band= %lf op_critical_path= %le op_off_critical_path= %le ratio_critical_path= %lf\n",
            0.0, band_size, (double)total, (double)st[0], (double)st[1], st[1] ? (double)st[0] / st[1] : 0.0,
            (double)st[2], (double)st[3], st[3] ? (double)st[2] / st[3] : 0.0);
+    /* the per-executor counters sum to the total (hicma_parsec/testing_dpotrf.c:982-1006) */
+    unsigned long ex[4] = {0, 0, 0, 0}, sum_exec = 0, sum_tile = 0;
+    hicma_exec_op_stats(A, ex, 4);
+    for (int i = 0; i < 4; i++) sum_exec += ex[i];
+    for (size_t i = 0; i < (size_t)NT * NT; i++) sum_tile += tileops[i];
+    if (sum_exec != total || sum_tile != total || st[2] + st[3] != total) {
+        printf("-- operation counts do not add up: streams %lu tiles %lu total %lu\n", sum_exec, sum_tile, total);
+        ret |= 1;
+    }
     printf("[****] TIME(s) %12.5f : Total_critical_path_time %lf, potrf %lf, trsm %lf, syrk %lf\n\n", 0.0,
            cpt[0] + cpt[1] + cpt[2], cpt[0], cpt[1], cpt[2]);
-    if (check) {
-        double ld = 0;
-        hicma_logdet(A, &ld);
-        printf("log det = %.15e\n", ld);
-    }
     printf("R-LO %d %d %d %d   %d %d   %d %d %d %d   %d %s %.1e %g %d %g   %d %d %d %d %d %d %d %d   "
            "%g %d %d   %g %d %d   %g %d %d   %lf %lf   %lf %lf %lf %lf    %lf %lf %lf %lf %lf    "
            "%le %le %le %le %le %lf %lf %d  hicma-mi355x \"",
@@ -235,6 +304,7 @@ int main(int argc, char **argv)
     printf("\"\n");
     free(G);
     free(tileops);
+    free(opc);
     hicma_desc_destroy(A);
     hicma_fini(ctx);
     return ret;

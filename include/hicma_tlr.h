@@ -18,6 +18,7 @@
  */
 #ifndef HICMA_TLR_H
 #define HICMA_TLR_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -46,10 +47,36 @@ extern "C" {
 
 typedef struct hicma_ctx hicma_ctx;
 typedef struct hicma_tlr_desc hicma_tlr_desc;
+/* dense N x N column-major matrix in HBM (the reference's dense dcA0 / dcAd
+ * descriptors of the --check, hicma_parsec/testing_dpotrf.c:640-665) */
+typedef struct hicma_dense hicma_dense;
+
+/* The STARS-H problem arguments of STARSH_gen / parsec_band_regenerate
+ * (hicma_parsec/hicma_parsec.h:108-121, 186-198: void *data,
+ * STARSH_kernel *kernel, STARSH_int *index).  Each may be NULL, which keeps
+ * what the descriptor was created with:
+ *   data   -> const hicma_kernel_params * (sigma2, beta, nugget)
+ *   kernel -> one of HICMA_KIND_* with the descriptor's point dimension
+ *             (a STARS-H kernel is a host function; here the covariance runs
+ *             on the device, so the kernel is named, not passed)
+ *   index  -> N point indices: matrix row i uses point index[i] of the point
+ *             set as created (a permutation of 0..N-1) */
+typedef struct { double sigma2, beta, nugget; } hicma_kernel_params;
+typedef int hicma_kernel;
+typedef int64_t hicma_index;
 
 /* ---- context and descriptors (replace setup_parsec / descriptor init,
  *      hicma_parsec/hicma_parsec.c:337-427, testing_dpotrf.c:419-453) ---- */
 int hicma_init(hicma_ctx **ctx, int device);
+/* a context with options, for several factorizations sharing one device:
+ * HICMA_CTX_SINGLE_STREAM puts all of its work on one stream (one hardware
+ * queue), HICMA_CTX_STREAM_ORDERED_ALLOC allocates and frees in stream order
+ * (hipMallocAsync), so freeing does not wait for other contexts' work, and
+ * cu_share > 0 sizes its persistent grids and workspaces for that many CUs
+ * instead of the whole device (results do not depend on it) */
+#define HICMA_CTX_SINGLE_STREAM 1
+#define HICMA_CTX_STREAM_ORDERED_ALLOC 2
+int hicma_init_ex(hicma_ctx **ctx, int device, int flags, int cu_share);
 int hicma_fini(hicma_ctx *ctx);
 /* creates the descriptor, generates the point set (jittered grid, Morton
  * order, seeded) and allocates tile slots of capacity maxrank in HBM */
@@ -65,8 +92,8 @@ int hicma_points(int N, int ndim, uint64_t seed, double *xyz);
 /* ---- STARSH_gen (hicma_parsec/hicma_parsec.h:186-198, STARSH_gen.jdf:342-369):
  * dense band tiles + randomized-SVD compression of off-band tiles on the GPU.
  * *info = -1 when a tile is not compressible under maxrank. */
-int STARSH_gen(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr_desc *Ar, double tol, int maxrank,
-               int band_size, int send_full_tile, int *info);
+int STARSH_gen(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr_desc *Ar, void *data, hicma_kernel *kernel,
+               hicma_index *index, double tol, int maxrank, int band_size, int send_full_tile, int *info);
 
 /* ---- rank map (hicma_parsec/hicma_parsec.h:146-166) ---- */
 /* parsec_rank_gather: rank_array[n*NT+m] for m-n >= band, 0 elsewhere */
@@ -83,14 +110,19 @@ int parsec_band_size_auto_tuning(hicma_ctx *ctx, hicma_tlr_desc *Ar, hicma_tlr_d
                                  int *rank_array, int NB);
 /* the same decision on a host rank map, without a descriptor */
 int hicma_band_autotune(const int *rank_array, int NT, int NB);
-int parsec_band_regenerate(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, int band_size);
+/* parsec_band_regenerate (hicma_parsec/hicma_parsec.h:108-121): dense band
+ * tiles regenerated in HBM; data / kernel / index as for STARSH_gen */
+int parsec_band_regenerate(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, void *data, hicma_kernel *kernel,
+                           hicma_index *index, int band_size);
 
 /* ---- the factorization (hicma_parsec/hicma_parsec.h:38-56) ----
  * A is factored in place: band tiles hold L, off-band tiles hold L = U V^T.
  * Returns info: 0 success, > 0 global 1-based row of the first non-positive
  * pivot (k*NB + iinfo), < 0 argument error (codes above).
  * tileopcounters: NT*NT (index m*NT+n), opcounters: >= 1 entry,
- * critical_path_time: 6 entries (potrf, trsm, syrk GPU seconds; 3..5 zero);
+ * critical_path_time: 6 entries: summed device seconds of potrf(k), trsm(k+1,k)
+ * and syrk(k,k+1) (each a launch group on its own stream), 3..5 zero; pass
+ * NULL unless the times are wanted (the events cost a little);
  * any of the three may be NULL. */
 int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr_desc *Ar, hicma_tlr_desc *Rank,
                    double acc, int fixed_rank, int maxrank, int *lookahead, int band_size, int hmb,
@@ -103,6 +135,23 @@ unsigned long hicma_op_counts(char op, unsigned long a, unsigned long b, unsigne
 int hicma_op_stats(const hicma_tlr_desc *A, unsigned long *stats);
 /* number of recompressions whose rank was capped at maxrank in the last call */
 int hicma_rank_overflows(const hicma_tlr_desc *A);
+/* number of recompression SVDs that stopped at the sweep limit in the last call */
+int hicma_jacobi_unconverged(const hicma_tlr_desc *A);
+/* op counts of the last call per executing stream (the reference's
+ * per-thread opcounters, hicma_parsec/testing_dpotrf.c:920-990): ops[0..3] =
+ * context stream, two recompression side streams, lookahead stream; entries
+ * n > 4 are 0.  They sum to the total. */
+int hicma_exec_op_stats(const hicma_tlr_desc *A, unsigned long *ops, int n);
+/* record the initial / minimum / maximum / final rank of every off-band tile
+ * in the following factorizations (the reference's dcRank) */
+int hicma_rank_history(hicma_tlr_desc *A, int enable);
+/* the recorded history: h[4*(n*NT+m) + q], q = initial, min, max, final
+ * (0 for band tiles) */
+int hicma_get_rank_history(const hicma_tlr_desc *A, int *h);
+/* parsec_rank_print (hicma_parsec/hicma_parsec.h:168-177, rank_print.jdf):
+ * prints the five rank maps of the last factorization to stdout; needs
+ * hicma_rank_history(Rank, 1) before it */
+int parsec_rank_print(hicma_ctx *ctx, hicma_tlr_desc *Rank, int band_size);
 /* low-rank gemm + recompression totals of the last call: op count (op_counts
  * formulas), number of tasks, algorithmic bytes (8 * words read + written) */
 int hicma_recompress_stats(const hicma_tlr_desc *A, unsigned long *stats);
@@ -113,10 +162,35 @@ int hicma_profile_read(hicma_ctx *ctx, double *ms_total, int *launches);
 /* copy tiles, band and rank map of src into dst (same N, NB, maxrank, kind) */
 int hicma_desc_copy(hicma_tlr_desc *dst, const hicma_tlr_desc *src);
 
-/* ---- STARSH_check (hicma_parsec/hicma_parsec.h:168-184): uncompress the
- * lower triangle into a host N x N column-major array (upper zeroed) ---- */
-int STARSH_check(hicma_ctx *ctx, int uplo, double *A0, hicma_tlr_desc *A, hicma_tlr_desc *Ar, int band_size,
+/* ---- the --check of the reference harness, on the device ---- */
+int hicma_dense_create(hicma_ctx *ctx, hicma_dense **D, int N);
+int hicma_dense_destroy(hicma_dense *D);
+/* host copies of the whole N x N array */
+int hicma_dense_get(const hicma_dense *D, double *host);
+int hicma_dense_set(hicma_dense *D, const double *host);
+/* exact covariance of the problem of a TLR descriptor, lower triangle (strict
+ * upper zeroed): starsh_generate_map of hicma_parsec/testing_dpotrf.c:651 */
+int hicma_dense_generate(hicma_ctx *ctx, int uplo, hicma_dense *D, const hicma_tlr_desc *problem);
+/* STARSH_check (hicma_parsec/hicma_parsec.h:168-184, STARSH_check.jdf:90-143):
+ * uncompress the lower triangle of A into the dense device matrix A0, strict
+ * upper triangle zeroed */
+int STARSH_check(hicma_ctx *ctx, int uplo, hicma_dense *A0, hicma_tlr_desc *A, hicma_tlr_desc *Ar, int band_size,
                  int maxrank, int *info);
+/* dense Cholesky of the checker (dplasma_dpotrf, hicma_parsec/testing_dpotrf.c:766),
+ * rocSOLVER with 64-bit indices; *info = LAPACK info */
+int hicma_dense_potrf(hicma_ctx *ctx, int uplo, hicma_dense *D, int *info);
+/* dplasma_dlansy(Frobenius, lower) of D, and of A - B */
+int hicma_dense_lansy(hicma_ctx *ctx, int uplo, const hicma_dense *D, double *norm);
+int hicma_dense_diff_lansy(hicma_ctx *ctx, int uplo, const hicma_dense *A, const hicma_dense *B, double *norm);
+/* 2 sum log L(i,i) of a dense factor */
+int hicma_dense_logdet(hicma_ctx *ctx, const hicma_dense *L, double *logdet);
+/* check_dpotrf2 / check_diff (hicma_parsec/testing_dpotrf.c:111-230): return 0
+ * (correct) or 1 (suspicious), the ratio in *result; verbose prints the
+ * reference's report */
+int hicma_check_dpotrf2(hicma_ctx *ctx, int verbose, int uplo, const hicma_dense *L, const hicma_dense *A0,
+                        double threshold, double *result);
+int hicma_check_diff(hicma_ctx *ctx, int verbose, int uplo, const hicma_dense *L, const hicma_dense *L0,
+                     double threshold, double *result);
 
 /* ---- tile access (host copies) ---- */
 /* band tile: NB*NB doubles, returns NB in *rank; off-band: NB*2*r doubles */
@@ -125,9 +199,61 @@ int hicma_set_tile(hicma_tlr_desc *A, int m, int n, const double *buf, int rank)
 /* whole rank map, NT*NT, as stored on the device */
 int hicma_get_rank_map(const hicma_tlr_desc *A, int *G);
 
-/* ---- Gaussian log-likelihood pieces on the factor ---- */
+/* ---- Gaussian log-likelihood pieces on the factor ----
+ * No counterpart in the reference, which stops at the factor; these consume
+ * the factor in the layout HiCMA_dpotrf_L leaves. */
 /* 2 * sum(log(diag(L))) */
 int hicma_logdet(hicma_tlr_desc *A, double *logdet);
+/* triangular solves with the TLR factor, B in DEVICE memory (N x nrhs, column
+ * major, ldb >= N), in place: trans 0: B <- L^-1 B, trans 1: B <- L^-T B */
+int hicma_tlr_trsm(hicma_ctx *ctx, hicma_tlr_desc *A, int trans, int nrhs, double *B, int ldb);
+/* B <- Sigma^-1 B = L^-T L^-1 B (dpotrs) */
+int hicma_tlr_potrs(hicma_ctx *ctx, hicma_tlr_desc *A, int nrhs, double *B, int ldb);
+/* Gaussian log-likelihood of z (DEVICE, N entries, overwritten by L^-1 z):
+ * logdet = log det Sigma, quad = z^T Sigma^-1 z,
+ * loglik = -(quad + logdet + N log 2 pi) / 2; any output may be NULL */
+int hicma_tlr_loglik(hicma_ctx *ctx, hicma_tlr_desc *A, double *z, double *logdet, double *quad, double *loglik);
+
+/* ---- serving: continuous batching of Gaussian log-likelihood requests ----
+ * No counterpart in the reference.  Each request is a statistics-2d-sqexp
+ * problem of its own (N points of the seeded jittered grid, kernel
+ * parameters, vector z) answered with log det Sigma and z^T Sigma^-1 z:
+ * STARSH_gen, band auto-tuning, HiCMA_dpotrf_L and the forward solve on the
+ * device, the factor never leaving HBM.  `slots` workers (one context, one
+ * stream and ncu/slots CUs each) run requests concurrently; a request is
+ * admitted when a worker is free and its HBM footprint fits the budget
+ * (FIFO; a later request may pass a head that does not fit, for at most 1 s).
+ * A request's result does not depend on what else runs. */
+typedef struct hicma_server hicma_server;
+typedef struct {
+    int N, NB, maxrank;           /* maxrank <= NB/2; 0 selects NB/2 */
+    double eps;                   /* accuracy (STARSH_gen and recompression) */
+    double sigma2, beta, nugget;  /* C(r) = sigma2 exp(-r^2 / (2 beta^2)) + nugget on the diagonal */
+    uint64_t seed;                /* point set */
+    const double *z;              /* N values (host, copied at submission), or NULL: z ~ N(0, I) from zseed */
+    uint64_t zseed;
+} hicma_lik_request;
+typedef struct {
+    int status;                   /* 0, else the failing step's info or HICMA_ERR_* */
+    int band;                     /* auto-tuned band size */
+    double logdet, quad, loglik;  /* log det Sigma, z^T Sigma^-1 z, -(quad + logdet + N log 2 pi) / 2 */
+    double t_queue, t_run, t_total; /* seconds: submission to start, start to end, submission to end */
+    size_t bytes;                 /* HBM reserved */
+} hicma_lik_result;
+int hicma_server_create(hicma_server **s, int device, int slots, double hbm_budget_gb);
+int hicma_server_submit(hicma_server *s, const hicma_lik_request *r, uint64_t *ticket);
+/* blocks until the request is done */
+int hicma_server_wait(hicma_server *s, uint64_t ticket, hicma_lik_result *res);
+/* runs what was submitted to the end, then stops */
+int hicma_server_destroy(hicma_server *s);
+/* z ~ N(0, I) of the requests' zseed (host) */
+int hicma_normal(uint64_t seed, int n, double *z);
+
+/* ---- device memory plumbing for callers without a GPU runtime of their own ---- */
+int hicma_dev_malloc(hicma_ctx *ctx, size_t bytes, void **p);
+int hicma_dev_free(hicma_ctx *ctx, void *p);
+/* blocking copy on the context's device: to_device 1 = host -> device, 0 = device -> host */
+int hicma_memcpy(hicma_ctx *ctx, void *dst, const void *src, size_t bytes, int to_device);
 
 #ifdef __cplusplus
 }

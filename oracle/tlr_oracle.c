@@ -863,6 +863,58 @@ double orc_dense_logdet(int N, const double *L)
     return 2.0 * s;
 }
 
+/* ---- triangular solves with the TLR factor ----
+ * No counterpart in the reference (it stops at the factor); this is the
+ * textbook blocked substitution over the A0 tile layout, the checker of the
+ * device solve.  trans = 0: B <- L^{-1} B, trans = 1: B <- L^{-T} B, with
+ * L_mk = U V^T for off-band tiles.  B is N x nrhs, column major, ld ldb. */
+int orc_tlr_trsm(const orc_tlr *t, int trans, int nrhs, double *B, int ldb)
+{
+    const int NT = t->NT, NB = t->NB;
+    const double one = 1.0;
+    if (nrhs <= 0) return 0;
+    double *W = malloc(sizeof(double) * (size_t)NB * nrhs);
+    for (int s = 0; s < NT; s++) {
+        const int k = trans ? NT - 1 - s : s;
+        const int kk = tdim(t, k);
+        double *Bk = B + (size_t)k * NB;
+        dtrsm_("L", "L", trans ? "T" : "N", "N", &kk, &nrhs, &one, t->tile[TIX(t, k, k)], &NB, Bk, &ldb);
+        const int lo = trans ? 0 : k + 1, hi = trans ? k : NT;
+        for (int o = lo; o < hi; o++) {
+            const int m = trans ? k : o, n = trans ? o : k;  /* tile L(m, n) */
+            const int mm = tdim(t, m), nn = tdim(t, n);
+            const double *T = t->tile[TIX(t, m, n)];
+            double *Bo = B + (size_t)o * NB;
+            if (m - n < t->band) {
+                if (trans) gemm('T', 'N', nn, nrhs, mm, -1.0, T, NB, Bk, ldb, 1.0, Bo, ldb);
+                else gemm('N', 'N', mm, nrhs, nn, -1.0, T, NB, Bk, ldb, 1.0, Bo, ldb);
+            } else {
+                const int r = t->rank[TIX(t, m, n)];
+                if (r <= 0) continue;
+                const double *U = T, *V = T + (size_t)NB * r;
+                if (trans) { /* B_n -= V (U^T B_k) */
+                    gemm('T', 'N', r, nrhs, mm, 1.0, U, NB, Bk, ldb, 0.0, W, r);
+                    gemm('N', 'N', nn, nrhs, r, -1.0, V, NB, W, r, 1.0, Bo, ldb);
+                } else { /* B_m -= U (V^T B_k) */
+                    gemm('T', 'N', r, nrhs, nn, 1.0, V, NB, Bk, ldb, 0.0, W, r);
+                    gemm('N', 'N', mm, nrhs, r, -1.0, U, NB, W, r, 1.0, Bo, ldb);
+                }
+            }
+        }
+    }
+    free(W);
+    return 0;
+}
+
+/* ---- CPU-baseline helpers: thread count and BLAS identification ---- */
+extern void mkl_get_version_string(char *buf, int len);
+int orc_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
+void orc_blas_version(char *buf, int len) { mkl_get_version_string(buf, len); }
+
 /* ---- tile access ---- */
 int orc_get_rank(const orc_tlr *t, int m, int n) { return t->rank[TIX(t, m, n)]; }
 

@@ -77,6 +77,15 @@ double orc_check_diff(int N, const double *L, const double *L0);
 int  orc_dense_potrf(int N, double *A);
 double orc_dense_logdet(int N, const double *L);
 
+/* triangular solves with the factor: trans 0: B <- L^{-1} B, 1: B <- L^{-T} B
+ * (B: N x nrhs, column major, ld ldb) */
+int  orc_tlr_trsm(const orc_tlr *t, int trans, int nrhs, double *B, int ldb);
+
+/* threads of the OpenMP task graph (returns the count in effect) and the
+ * BLAS library's version string, for the CPU baseline's report */
+int  orc_set_threads(int n);
+void orc_blas_version(char *buf, int len);
+
 /* tile access for comparisons / import of a device-generated matrix */
 int  orc_get_rank(const orc_tlr *t, int m, int n);
 int  orc_get_tile(const orc_tlr *t, int m, int n, double *buf);

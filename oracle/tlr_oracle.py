@@ -49,6 +49,9 @@ def lib():
             "orc_get_tile": (i, [vp, i, i, vp]),
             "orc_set_band": (i, [vp, i]),
             "orc_set_tile": (i, [vp, i, i, vp, i]),
+            "orc_set_threads": (i, [i]),
+            "orc_tlr_trsm": (i, [vp, i, i, vp, i]),
+            "orc_blas_version": (None, [C.c_char_p, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -56,6 +59,16 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    return lib().orc_set_threads(n)
+
+
+def blas_version():
+    buf = C.create_string_buffer(256)
+    lib().orc_blas_version(buf, 256)
+    return buf.value.decode(errors="replace").strip()
 
 
 def points(N, ndim, seed=0):
@@ -172,6 +185,14 @@ class Oracle:
         U = buf[: self.NB * r].reshape(self.NB, r, order="F")
         V = buf[self.NB * r: 2 * self.NB * r].reshape(self.NB, r, order="F")
         return (U, V), r
+
+    def trsm(self, B, trans=0):
+        """L^{-1} B (trans 0) or L^{-T} B (trans 1) with the factor held."""
+        X = np.array(B, dtype=np.float64, order="F", copy=True)
+        if X.ndim == 1:
+            X = X.reshape(-1, 1, order="F")
+        lib().orc_tlr_trsm(self.h, trans, X.shape[1], X.ctypes.data, X.shape[0])
+        return X.reshape(np.shape(B)) if np.ndim(B) == 1 else X
 
     def set_tile(self, m, n, data, rank):
         data = np.ascontiguousarray(data, dtype=np.float64)

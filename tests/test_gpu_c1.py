@@ -48,8 +48,9 @@ def test_factorization_check(ctx, exact, auto):
     band = 1
     if auto:
         band = g.auto_tune()
-        if band != 1:
-            g.regenerate(band)
+        # the dense band kernels (trsm, syrk, gemm cases i-iv) are reached only when band > 1
+        assert 1 < band < g.NT
+        g.regenerate(band)
     Aapprox = g.uncompress()
     info = g.potrf(EPS, 0, U)
     assert info == 0
@@ -66,3 +67,36 @@ def test_factorization_check(ctx, exact, auto):
     rel = abs(ld - ld_dense) / abs(ld_dense)
     print(f"logdet {ld!r} dense {ld_dense!r} rel {rel:.2e}")
     assert rel <= 1e-8
+
+
+def test_rank_check_history_and_counters(ctx):
+    """parsec_rank_check (rank_check.jdf:22-61), the rank history behind
+    parsec_rank_print (rank_print.jdf), the per-stream op counts and the
+    critical-path times of one factorization."""
+    g = H.TLRMatrix(ctx, N, NB, U, H.SQEXP_2D, 1.0, 0.1, 1e-2, 0)
+    assert g.generate(EPS, U, 1) == 0
+    before = g.rank_map().copy()
+    g.rank_history(True)
+    assert g.potrf(EPS, 0, U, want_ops=True, critical_path=True) == 0
+    after = g.rank_map().copy()
+    assert g.rank_check() == 0
+    R = g.rank_map()
+    for m in range(g.NT):
+        assert R[m, m] == -1                      # band entries set to -1
+        for n in range(m):
+            assert R[m, n] == after[m, n] >= 0
+    # the history: initial = before, final = after, min <= both <= max
+    h = g.get_rank_history()
+    for m in range(g.NT):
+        for n in range(m):
+            ini, lo, hi, fin = h[m, n]
+            assert ini == before[m, n] and fin == after[m, n]
+            assert lo <= min(ini, fin) and hi >= max(ini, fin)
+    assert (h[:, :, 1] < h[:, :, 2]).any()        # ranks changed somewhere during the factorization
+    g.rank_print()
+    st, ex = g.stats, g.exec_stats
+    assert int(ex.sum()) == int(st[0] + st[1]) == int(g.tileops.sum())
+    assert (ex > 0).sum() >= 2
+    cpt = g.critical_path_time
+    assert cpt[0] > 0 and cpt[1] > 0 and cpt[2] > 0 and np.all(cpt[3:] == 0)
+    g.close()

@@ -32,3 +32,44 @@ def test_c2_rerun_is_bit_identical(ctx):
     assert abs(la - ref) <= 1e-9 * abs(ref)
     for T in (A, B):
         T.close()
+
+
+def test_c2_device_check(ctx):
+    """The reference's --check at the benchmark size, on the device: the TLR
+    factor against the dense Cholesky of the approximate matrix (check_diff),
+    L L^T against the exact matrix (check_dpotrf2), and the log-determinant
+    against the dense FP64 Cholesky of the exact matrix.  Two dense N x N
+    matrices (2 x 64.8 GB) live in HBM beside the TLR matrix."""
+    import time
+    A = H.TLRMatrix(ctx, N, NB, U, H.SQEXP_2D, 1.0, 0.1, 1e-2, 0)
+    assert A.generate(EPS, U, 1) == 0
+    band = A.auto_tune()
+    if band != 1:
+        A.regenerate(band)
+    D1 = H.Dense(ctx, N)
+    D2 = H.Dense(ctx, N)
+    t0 = time.time()
+    D2.uncompress(A)
+    assert D2.potrf() == 0                      # L0 = chol(approximate A)
+    print(f"uncompress + dense potrf {time.time() - t0:.1f}s", flush=True)
+    assert A.potrf(EPS, 0, U) == 0
+    assert A.jacobi_unconverged() == 0 and A.rank_overflows() == 0
+    ld = A.logdet()
+    t0 = time.time()
+    D1.uncompress(A)                            # L
+    bad, diff = H.check_diff(ctx, D1, D2, EPS, verbose=1)
+    print(f"check_diff {diff:.3e} ({time.time() - t0:.1f}s)", flush=True)
+    assert bad == 0 and diff <= 10 * EPS
+    t0 = time.time()
+    D2.generate(A)                              # exact A
+    bad, res = H.check_dpotrf2(ctx, D1, D2, EPS, verbose=1)
+    print(f"check_dpotrf2 {res:.3e} ({time.time() - t0:.1f}s)", flush=True)
+    assert bad == 0 and res <= 10 * EPS
+    t0 = time.time()
+    assert D2.potrf() == 0                      # chol(exact A)
+    ldd = D2.logdet()
+    rel = abs(ld - ldd) / abs(ldd)
+    print(f"logdet TLR {ld!r} dense {ldd!r} rel {rel:.2e} ({time.time() - t0:.1f}s)", flush=True)
+    assert rel <= 1e-8
+    for X in (D1, D2, A):
+        X.close()

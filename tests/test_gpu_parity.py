@@ -35,14 +35,21 @@ def test_ragged_last_tile(ctx, N, NB, U):
     assert abs(g.logdet() - O.dense_logdet(Ld)) <= 1e-8 * abs(O.dense_logdet(Ld))
 
 
-@pytest.mark.parametrize("kind,beta", [(H.EXP_3D, 0.1), (H.SQEXP_3D, 0.05)])
-def test_3d_kernels(ctx, kind, beta):
+# 3-D exp at this size auto-tunes to a fully dense band (8 = NT), so it runs
+# with band 1 (every off-diagonal tile low rank, ranks 101..214: stacks of up
+# to ~430 columns); 3-D sqexp auto-tunes to a band between 1 and NT
+@pytest.mark.parametrize("kind,beta,auto", [(H.EXP_3D, 0.1, False), (H.SQEXP_3D, 0.05, True)])
+def test_3d_kernels(ctx, kind, beta, auto):
     N, NB, U = 4096, 512, 256
-    g, info, band, L, A = factor_and_check(ctx, N, NB, U, kind, beta)
+    g, info, band, L, A = factor_and_check(ctx, N, NB, U, kind, beta, auto=auto)
     assert info == 0
+    assert band < g.NT
+    assert g.jacobi_unconverged() == 0
     res = O.check_dpotrf2(L, A)
     print(f"kind={kind} band={band} residual={res:.3e}")
     assert res <= 10 * EPS
+    Ld, _ = O.dense_potrf(A)
+    assert abs(g.logdet() - O.dense_logdet(Ld)) <= 1e-8 * abs(O.dense_logdet(Ld))
 
 
 def test_fixed_rank_mode(ctx):
