@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(256) k_sum_seq(const double *part, int n, doub
     if (threadIdx.x == 0) *out = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// 2 sum log L(i, i) over the true N (same order as k_logdet)
+// 2 sum log L(i, i) over the true N, in the order of k_logdet (hicma_logdet)
 __global__ void __launch_bounds__(256) k_logdet_tiles(TLR t, double *out)
 {
     __shared__ double red[4];
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(256) k_logdet_tiles(TLR t, double *out)
     part = wsum(part);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
     __syncthreads();
-    if (threadIdx.x == 0) *out = 2.0 * ((red[0] + red[1]) + (red[2] + red[3]));
+    if (threadIdx.x == 0) *out = 2.0 * (((red[0] + red[1]) + red[2]) + red[3]);
 }
 
 // ---------------------------------------------------------------------------

@@ -22,7 +22,10 @@ def test_c3_shape_exp3d(ctx):
     print(f"ranks {rmin}..{rmax}, band {band}", flush=True)
     assert 1 < band < A.NT
     A.regenerate(band)
+    import time
+    t0 = time.time()
     assert A.potrf(EPS, 0, U) == 0
+    print(f"factorization {time.time() - t0:.1f}s", flush=True)
     assert A.jacobi_unconverged() == 0 and A.rank_overflows() == 0
     ld = A.logdet()
     L = H.Dense(ctx, N)
