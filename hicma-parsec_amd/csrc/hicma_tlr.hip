@@ -144,6 +144,7 @@ extern "C" int hicma_init_ex(hicma_ctx **ctx, int device, int flags, int cu_shar
     HIPCHK(hipMemset(c->d_info, 0, 4 * sizeof(int)));
     c->d_ovf = c->d_info + 1;
     HIPCHK(hipMalloc(&c->d_scalar, 8 * sizeof(double)));
+    RCHK(hicma_noalloc_setup(device));
     *ctx = c;
     return 0;
 }

@@ -87,6 +87,9 @@ hipError_t ctx_free(hicma_ctx *c, void *p);
 // factorization of that shape allocates on a context (hicma_tlr.hip)
 size_t hicma_desc_bytes(int N, int NB, int cap, int band, int ndim);
 size_t hicma_work_bytes(hicma_ctx *ctx, int N, int NB, int cap, int band);
+// device address of the allocation-free entry point's scratch array, fetched
+// once per device (hicma_noalloc.hip)
+int hicma_noalloc_setup(int device);
 // the solve's launch sequence and its helpers (hicma_solve.hip)
 size_t solve_ws_doubles(const TLR &t, int nrhs);
 int check_solve_args(hicma_ctx *ctx, hicma_tlr_desc *A, int nrhs, const double *B, long ldb);

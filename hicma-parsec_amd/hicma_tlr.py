@@ -31,7 +31,7 @@ EXPORTS = [
     "hicma_rank_history", "hicma_get_rank_history", "parsec_rank_print", "hicma_dense_create", "hicma_dense_destroy", "hicma_dense_get",
     "hicma_dense_set", "hicma_dense_generate", "hicma_dense_potrf", "hicma_dense_lansy", "hicma_dense_diff_lansy",
     "hicma_dense_logdet", "hicma_check_dpotrf2", "hicma_check_diff", "hicma_tlr_trsm", "hicma_tlr_potrs",
-    "hicma_tlr_loglik", "hicma_dev_malloc", "hicma_dev_free", "hicma_memcpy",
+    "hicma_tlr_loglik", "hicma_dev_malloc", "hicma_dev_free", "hicma_memcpy", "hicma_tlr_loglik_noalloc",
     "hicma_init_ex", "hicma_server_create", "hicma_server_submit", "hicma_server_wait", "hicma_server_destroy",
     "hicma_normal",
 ]
@@ -114,6 +114,7 @@ def lib():
             "hicma_dev_malloc": (i, [vp, C.c_size_t, C.POINTER(vp)]),
             "hicma_dev_free": (i, [vp, vp]),
             "hicma_memcpy": (i, [vp, vp, vp, C.c_size_t, i]),
+            "hicma_tlr_loglik_noalloc": (i, [vp, vp, vp, C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
             "hicma_init_ex": (i, [C.POINTER(vp), i, i, i]),
             "hicma_server_create": (i, [C.POINTER(vp), i, i, d]),
             "hicma_server_submit": (i, [vp, C.POINTER(LikRequest), C.POINTER(C.c_uint64)]),

@@ -214,6 +214,15 @@ int hicma_tlr_potrs(hicma_ctx *ctx, hicma_tlr_desc *A, int nrhs, double *B, int 
  * loglik = -(quad + logdet + N log 2 pi) / 2; any output may be NULL */
 int hicma_tlr_loglik(hicma_ctx *ctx, hicma_tlr_desc *A, double *z, double *logdet, double *quad, double *loglik);
 
+/* hicma_tlr_loglik without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * serving loops).  Its workspace is a 64 MiB device array of the library
+ * (8 Mi doubles: the diagonal-block inverses, NT*NB*(NB+63)/64*64 doubles, plus
+ * NT*NB + NT*NB/256 + 8); larger problems return HICMA_ERR_ARG.  Calls that
+ * overlap on one GPU race on that array. */
+int hicma_tlr_loglik_noalloc(hicma_ctx *ctx, hicma_tlr_desc *A, double *z, double *logdet, double *quad,
+                             double *loglik);
+
 /* ---- serving: continuous batching of Gaussian log-likelihood requests ----
  * No counterpart in the reference.  Each request is a statistics-2d-sqexp
  * problem of its own (N points of the seeded jittered grid, kernel
