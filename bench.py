@@ -139,6 +139,7 @@ def serving_sweep(rates, nreq, slots, budget_gb, seed=2024):
     rows = []
     with H.Server(0, slots=slots, hbm_budget_gb=budget_gb) as srv:
         srv.wait(srv.submit(4096, 1.0, 0.1, 1e-2, seed=0, zseed=0))  # first-use costs out of the way
+        log("serving: warm-up request done")
         rid = 1
         for rate in rates:
             reqs = []
@@ -157,7 +158,11 @@ def serving_sweep(rates, nreq, slots, budget_gb, seed=2024):
                     time.sleep(dt)
                 sub[i] = time.perf_counter()
                 tk = srv.submit(N, s2, b, 1e-2, seed=sd, zseed=sd)
-                th = threading.Thread(target=lambda i=i, tk=tk: res.__setitem__(i, srv.wait(tk)))
+                def waiter(i=i, tk=tk, N=N):
+                    res[i] = srv.wait(tk)
+                    log(f"  rate {rate}/s request {i} N={N}: status {res[i]['status']}, "
+                        f"latency {res[i]['t_total']:.3f}s (queued {res[i]['t_queue']:.3f}s)")
+                th = threading.Thread(target=waiter)
                 th.start()
                 threads.append(th)
             for th in threads:
@@ -252,9 +257,13 @@ def main():
         log(f"cpu baseline {cpu['value']}s on {cpu['cores']} threads ({cpu['cpu']})")
 
     serving = None
+    two_flow = int(A.two_flow)
     if not a.no_serve:
+        # the server's slots get their own streams: free this context's first
+        # (streams beyond the device's hardware queues share one and serialize)
         A.close()
         pristine.close()
+        ctx.close()
         rates = [float(x) for x in a.serve_rates.split(",") if x]
         serving = {"slots": a.serve_slots, "hbm_budget_gb": 200.0,
                    "workload": ("Poisson arrivals; N = 512 k, k ~ U{8..128}; sigma2 ~ U[0.5, 2], beta ~ U[0.02, 0.2], "
@@ -278,7 +287,7 @@ def main():
             "workload": (f"HiCMA_dpotrf_L, statistics-2d-sqexp N={a.N} (jittered grid, Morton order, seed 0), "
                          f"NB={a.NB}, eps={a.eps:g}, maxrank={a.maxrank}, beta=0.1, nugget=1e-2, "
                          f"band auto-tuned={band}"),
-            "N": a.N, "NB": a.NB, "maxrank": a.maxrank, "band_size": band, "two_flow": int(A.two_flow),
+            "N": a.N, "NB": a.NB, "maxrank": a.maxrank, "band_size": band, "two_flow": two_flow,
             "ranks_initial": [rmin, round(ravg, 2), rmax], "ranks_final": [fmin, round(favg, 2), fmax],
             "op_count": total_ops, "logdet": logdet, "generation_s": round(tgen, 3),
         },

@@ -52,7 +52,8 @@ double now_s()
 struct hicma_server {
     int device, nslots, cu_share;
     size_t budget, reserved;
-    hicma_ctx *probe;  // context parameters for footprint estimates
+    std::vector<hicma_ctx *> ctxs;  // one single-stream context per slot
+    hicma_ctx *probe;  // ctxs[0]: its (read-only) parameters size the footprints
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<SvRequest *> queue;
@@ -60,7 +61,6 @@ struct hicma_server {
     uint64_t next_id;
     int growing;   // requests waiting for band memory
     int running;   // admitted, not finished
-    int started, init_err;
     bool stop;
     std::vector<std::thread> workers;
 };
@@ -174,18 +174,8 @@ static hicma_lik_result run(hicma_server *S, hicma_ctx *ctx, SvRequest *q)
     return res;
 }
 
-static void worker(hicma_server *S)
+static void worker(hicma_server *S, hicma_ctx *ctx)
 {
-    hicma_ctx *ctx = nullptr;
-    const int rc = hicma_init_ex(&ctx, S->device, HICMA_CTX_SINGLE_STREAM | HICMA_CTX_STREAM_ORDERED_ALLOC,
-                                 S->cu_share);
-    {
-        std::lock_guard<std::mutex> lk(S->mu);
-        S->started++;
-        if (rc) S->init_err = rc;
-        S->cv_done.notify_all();
-    }
-    if (rc) return;
     for (;;) {
         SvRequest *q = nullptr;
         {
@@ -203,40 +193,36 @@ static void worker(hicma_server *S)
         }
         delete q;
     }
-    hicma_fini(ctx);
 }
 
 extern "C" int hicma_server_create(hicma_server **out, int device, int slots, double hbm_budget_gb)
 {
     if (!out || slots < 1 || slots > 32 || !(hbm_budget_gb > 0)) return HICMA_ERR_ARG;
+    int ncu = 0;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || ncu < 1)
+        return HICMA_ERR_HIP;
     hicma_server *S = new hicma_server();
     S->device = device;
     S->nslots = slots;
     S->budget = (size_t)(hbm_budget_gb * 1e9);
     S->next_id = 1;
-    int rc = hicma_init(&S->probe, device);
-    if (rc) {
-        delete S;
-        return rc;
+    S->cu_share = ncu / slots > 0 ? ncu / slots : 1;
+    // every context is created here, so the process holds exactly one stream
+    // per slot (the device offers few hardware queues: streams beyond them
+    // share one and serialize)
+    for (int i = 0; i < slots; i++) {
+        hicma_ctx *c = nullptr;
+        const int rc = hicma_init_ex(&c, device, HICMA_CTX_SINGLE_STREAM | HICMA_CTX_STREAM_ORDERED_ALLOC,
+                                     S->cu_share);
+        if (rc) {
+            hicma_server_destroy(S);
+            return rc;
+        }
+        S->ctxs.push_back(c);
     }
-    S->cu_share = S->probe->ncu / slots > 0 ? S->probe->ncu / slots : 1;
-    hicma_fini(S->probe);
-    S->probe = nullptr;
-    rc = hicma_init_ex(&S->probe, device, HICMA_CTX_SINGLE_STREAM, S->cu_share);
-    if (rc) {
-        delete S;
-        return rc;
-    }
-    for (int i = 0; i < slots; i++) S->workers.emplace_back(worker, S);
-    {
-        std::unique_lock<std::mutex> lk(S->mu);
-        S->cv_done.wait(lk, [&] { return S->started == slots; });
-        rc = S->init_err;
-    }
-    if (rc) {
-        hicma_server_destroy(S);
-        return rc;
-    }
+    S->probe = S->ctxs[0];
+    for (int i = 0; i < slots; i++) S->workers.emplace_back(worker, S, S->ctxs[i]);
     *out = S;
     return 0;
 }
@@ -285,7 +271,7 @@ extern "C" int hicma_server_destroy(hicma_server *S)
         S->cv_work.notify_all();
     }
     for (auto &t : S->workers) t.join();
-    if (S->probe) hicma_fini(S->probe);
+    for (hicma_ctx *c : S->ctxs) hicma_fini(c);
     for (SvRequest *q : S->queue) delete q;
     delete S;
     return 0;

@@ -75,6 +75,8 @@ static void sketch(int ld, int ncol, uint64_t seed, double *om)
     }
 }
 #define SKETCH_SEED 0x5EED0FULL
+// width of the first generation sketch (ranks up to 54 are settled by it)
+#define GEN_FIRST_SKETCH 64
 
 // z ~ N(0, I): the same counter-based normal stream as the sketch
 extern "C" int hicma_normal(uint64_t seed, int n, double *z)
@@ -270,6 +272,8 @@ extern "C" int hicma_desc_destroy(hicma_tlr_desc *A)
     ctx_free(c, A->d_tileops);
     ctx_free(c, A->d_opstat);
     ctx_free(c, A->rhist);
+    if (A->gexec) (void)hipGraphExecDestroy(A->gexec);
+    if (A->graph) (void)hipGraphDestroy(A->graph);
     free(A->h_xyz);
     free(A->h_xyz0);
     free(A);
@@ -396,8 +400,13 @@ extern "C" int STARSH_gen(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr
         if ((rc = hicma_ensure_ws(ctx, g.stride * sizeof(double) * ns))) return rc;
         size_t lds = sizeof(double) * (BLK_GEMM_SMEM + BLK_RED_SMEM + 2 * QRB * QRB + 3 * (size_t)p + 16);
         HIPCHK(hipFuncSetAttribute((const void *)k_compress, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        // narrow sketch first; tiles whose rank it cannot vouch for again at full width
+        const int pw = p < GEN_FIRST_SKETCH ? p : GEN_FIRST_SKETCH;
         hipLaunchKernelGGL(k_compress, dim3(ns), dim3(BS), lds, ctx->stream, t, (const double *)d_om, ctx->ws, g,
-                           tol, maxrank, ctx->d_info);
+                           pw, 0, tol, maxrank, ctx->d_info);
+        if (pw < p)
+            hipLaunchKernelGGL(k_compress, dim3(ns), dim3(BS), lds, ctx->stream, t, (const double *)d_om, ctx->ws,
+                               g, p, 1, tol, maxrank, ctx->d_info);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->stream));
         HIPCHK(ctx_free(ctx, d_om));
@@ -658,13 +667,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     hipStream_t st = ctx->stream;
     TLR &t = A->t;
     const int NT = t.NT, NB = t.NB, cap = t.cap, b = band_size;
-    HIPCHK(hipMemsetAsync(ctx->d_info, 0, 4 * sizeof(int), st));
-    HIPCHK(hipMemsetAsync(A->d_tileops, 0, sizeof(unsigned long long) * (size_t)NT * NT, st));
-    HIPCHK(hipMemsetAsync(A->d_opstat, 0, sizeof(unsigned long long) * 16, st));
-    if (A->rhist_on) {
-        if (!A->rhist) HIPCHK(ctx_malloc(ctx, (void **)&A->rhist, sizeof(int) * 4 * (size_t)NT * NT));
-        hipLaunchKernelGGL(k_rank_hist, dim3((NT * NT + 255) / 256), dim3(256), 0, st, t, A->rhist, 0);
-    }
+    if (A->rhist_on && !A->rhist) HIPCHK(ctx_malloc(ctx, (void **)&A->rhist, sizeof(int) * 4 * (size_t)NT * NT));
     // one op log per executing stream: [8] context stream, [9] side 0,
     // [10] side 1, [11] the lookahead stream
     OpLog o{A->d_tileops, A->d_opstat, A->d_opstat + 8, A->rhist_on ? A->rhist : nullptr, ctx->d_info + 2};
@@ -730,6 +733,47 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     auto cp = [&](int k, int slot, hipStream_t s) -> hipError_t {
         return timing ? hipEventRecord(ctx->cp_ev[6 * (size_t)k + slot], s) : hipSuccess;
     };
+    // HIP graph: the whole launch sequence below depends only on (NT, band)
+    // and on the pointers and scalars in the key, so it is captured once and
+    // replayed while they are unchanged (timing, profiling and rank-history
+    // runs are always launched directly)
+    const bool use_graph = A->use_graph && !timing && !ctx->profile && !A->rhist_on;
+    GraphKey key;
+    memset(&key, 0, sizeof(key));
+    key.band = t.band;
+    key.lr = t.lr;
+    key.rank = t.rank;
+    key.xyz = t.xyz;
+    key.ws = ctx->ws;
+    key.tileops = A->d_tileops;
+    key.opstat = A->d_opstat;
+    key.acc = acc;
+    key.sigma2 = t.sigma2;
+    key.beta = t.beta;
+    key.nugget = t.nugget;
+    key.fixed_rank = fixed_rank;
+    key.maxrank = maxrank;
+    key.band_size = b;
+    key.N = t.N;
+    key.NB = NB;
+    key.NT = NT;
+    key.cap = cap;
+    key.kind = t.kind;
+    const bool replay = use_graph && A->gexec && memcmp(&key, &A->gkey, sizeof(key)) == 0;
+    if (use_graph && !replay) {
+        if (A->gexec) (void)hipGraphExecDestroy(A->gexec);
+        if (A->graph) (void)hipGraphDestroy(A->graph);
+        A->gexec = nullptr;
+        A->graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    }
+    auto enqueue = [&]() -> int {
+    int rc = 0;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, ctx->d_info, 4, 0);
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)(((long)NT * NT + 255) / 256)), dim3(256), 0, st, A->d_tileops,
+                       (long)NT * NT);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, A->d_opstat, 16L);
+    if (A->rhist_on) hipLaunchKernelGGL(k_rank_hist, dim3((NT * NT + 255) / 256), dim3(256), 0, st, t, A->rhist, 0);
     const int ntg = (NB + GT - 1) / GT;
     const int ntl = ntg * (ntg + 1) / 2;
     // Lookahead: syrk(k) and potrf(k+1) touch only diagonal tiles and the gemm
@@ -784,7 +828,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
             }
             hipLaunchKernelGGL(k_classify, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k, tasks_s);
             if (wave) {
-                HIPCHK(hipMemsetAsync(wcount, 0, 8 * sizeof(int), st));
+                hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, wcount, 8, 0);
                 hipLaunchKernelGGL(k_classify_wave, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k,
                                    tasks_s, wlist, wcount, wcap);
                 // fork: variants 32 and 48 on the context stream, 64 and the
@@ -832,6 +876,27 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     }
     if (A->rhist_on)
         hipLaunchKernelGGL(k_rank_hist, dim3((NT * NT + 255) / 256), dim3(256), 0, st, t, A->rhist, 1);
+    HIPCHK(hipGetLastError());
+    return 0;
+    };
+    if (!replay) {
+        rc = enqueue();
+        if (use_graph) {
+            hipGraph_t gr = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &gr);
+            if (rc || ec != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                if (rc) return rc;
+                HIPCHK(ec);
+            }
+            A->graph = gr;
+            HIPCHK(hipGraphInstantiate(&A->gexec, A->graph, nullptr, nullptr, 0));
+            A->gkey = key;
+        } else if (rc) {
+            return rc;
+        }
+    }
+    if (use_graph) HIPCHK(hipGraphLaunch(A->gexec, st));
     HIPCHK(hipStreamSynchronize(st));
     int hinfo[4];
     HIPCHK(hipMemcpy(hinfo, ctx->d_info, 4 * sizeof(int), hipMemcpyDeviceToHost));
@@ -929,6 +994,13 @@ extern "C" int hicma_desc_copy(hicma_tlr_desc *dst, const hicma_tlr_desc *src)
 extern "C" int hicma_rank_overflows(const hicma_tlr_desc *A) { return A ? A->last_ovf : 0; }
 
 extern "C" int hicma_jacobi_unconverged(const hicma_tlr_desc *A) { return A ? A->last_unconv : 0; }
+
+extern "C" int hicma_factor_graph(hicma_tlr_desc *A, int enable)
+{
+    if (!A) return HICMA_ERR_ARG;
+    A->use_graph = enable ? 1 : 0;
+    return 0;
+}
 
 extern "C" int hicma_exec_op_stats(const hicma_tlr_desc *A, unsigned long *ops, int n)
 {

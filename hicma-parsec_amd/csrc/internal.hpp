@@ -51,6 +51,14 @@ struct hicma_ctx {
     void (*blas_free)(void *);
 };
 
+// what a captured factorization graph depends on: every pointer and scalar
+// its kernels take (a replay is valid only while all are unchanged)
+struct GraphKey {
+    const void *band, *lr, *rank, *xyz, *ws, *tileops, *opstat;
+    double acc, sigma2, beta, nugget;
+    int fixed_rank, maxrank, band_size, N, NB, NT, cap, kind;
+};
+
 struct hicma_tlr_desc {
     hicma_ctx *ctx;
     TLR t;  // device pointers + sizes
@@ -66,6 +74,10 @@ struct hicma_tlr_desc {
     int last_unconv;
     int *rhist;                     // NT*NT x {initial, min, max, final} rank history, or null
     int rhist_on;
+    int use_graph;                  // capture the factorization once, replay it after
+    hipGraph_t graph;
+    hipGraphExec_t gexec;
+    GraphKey gkey;
 };
 
 struct hicma_dense {

@@ -30,10 +30,23 @@ __host__ __device__ inline size_t gen_slot_doubles(int NB, int p, int cap)
            2 * (size_t)p + 2 * nt * QRB * QRB + 2 * (size_t)p * p + (size_t)p * cap + 64;
 }
 
+// oversampling of the randomized SVD (STARS-H's default)
+#define GEN_OVS 10
+// rank-map marker of a tile whose first, narrow sketch was too narrow
+#define GEN_REDO (-2)
+
 // STARS-H starsh_dense_dlrrsdd restated as a workgroup pipeline:
 // D = kernel tile; Y = D*Omega; Q = qr(Y); X = D^T Q; X = Qx Rx; Jacobi SVD of Rx;
 // rank by relative Frobenius tail <= tol; U = Q*Vr, V = Qx*(Ur*S).
-__global__ void __launch_bounds__(BS) k_compress(TLR t, const double *Om, double *ws, GenWs g,
+//
+// The sketch width is adaptive: the first pass (redo = 0) uses the first
+// pw <= g.p columns of Omega.  A tile whose rank comes out above
+// pw - GEN_OVS may have more than the sketch can see, so it is marked
+// GEN_REDO and the second pass (redo = 1) compresses exactly the marked
+// tiles with the full width g.p = maxrank + GEN_OVS, as STARS-H does for
+// every tile.  Ranks far below maxrank (the common case) thus cost a sketch
+// of pw columns instead of maxrank + 10.
+__global__ void __launch_bounds__(BS) k_compress(TLR t, const double *Om, double *ws, GenWs g, int pw, int redo,
                                                  double tol, int maxrank, int *info)
 {
     extern __shared__ double sm[];
@@ -42,25 +55,27 @@ __global__ void __launch_bounds__(BS) k_compress(TLR t, const double *Om, double
     int *pos = (int *)(ssig + g.p);
     int *flag = pos + g.p;
     int *rk = flag + 1;
-    const int NB = t.NB, NT = t.NT, p = g.p, tid = threadIdx.x;
+    // workspace layout for the full width P; this pass computes with p <= P
+    const int NB = t.NB, NT = t.NT, P = g.p, p = pw, tid = threadIdx.x;
     double *D = ws + (size_t)blockIdx.x * g.stride;
     double *Y = D + (size_t)NB * NB;
-    double *Qm = Y + (size_t)NB * p;
-    double *X = Qm + (size_t)NB * p;
-    double *bw = X + (size_t)NB * p;
-    double *tauY = bw + (size_t)NB * QRB + (size_t)QRB * (p > NB ? p : NB);
-    double *tauX = tauY + p;
-    size_t ntp = (size_t)(p + QRB - 1) / QRB;
-    double *TY = tauX + p;
+    double *Qm = Y + (size_t)NB * P;
+    double *X = Qm + (size_t)NB * P;
+    double *bw = X + (size_t)NB * P;
+    double *tauY = bw + (size_t)NB * QRB + (size_t)QRB * (P > NB ? P : NB);
+    double *tauX = tauY + P;
+    size_t ntp = (size_t)(P + QRB - 1) / QRB;
+    double *TY = tauX + P;
     double *TX = TY + ntp * QRB * QRB;
     double *Mj = TX + ntp * QRB * QRB;
-    double *Vj = Mj + (size_t)p * p;
-    double *Vsel = Vj + (size_t)p * p;
+    double *Vj = Mj + (size_t)P * P;
+    double *Vsel = Vj + (size_t)P * P;
     const long long ntask = (long long)NT * (NT - 1) / 2;
     for (long long task = blockIdx.x; task < ntask; task += gridDim.x) {
         int m, n;
         tri_decode(task, m, n);
         if (m - n < t.band_size) continue;
+        if (redo && *rank_at(t, m, n) != GEN_REDO) continue;
         // D
         for (size_t e = tid; e < (size_t)NB * NB; e += BS) {
             int i = (int)(e % NB), j = (int)(e / NB);
@@ -94,11 +109,17 @@ __global__ void __launch_bounds__(BS) k_compress(TLR t, const double *Om, double
             }
             int rows = tdim(t, m), cols = tdim(t, n);
             int mn = rows < cols ? rows : cols;
-            if (!(r < mn / 2 && r <= maxrank)) r = -1;
+            if (p < P && r > p - GEN_OVS) r = GEN_REDO;
+            else if (!(r < mn / 2 && r <= maxrank)) r = -1;
             *rk = r;
         }
         __syncthreads();
         const int r = *rk;
+        if (r == GEN_REDO) {
+            if (tid == 0) *rank_at(t, m, n) = GEN_REDO;
+            __syncthreads();
+            continue;
+        }
         if (r < 0) {
             if (tid == 0) { *rank_at(t, m, n) = -1; atomicExch(info, -1); }
             __syncthreads();
@@ -691,6 +712,18 @@ __global__ void k_rank_hist(TLR t, int *h, int phase)
         p[2] = r;
     }
     p[3] = r;
+}
+
+// fills that replace hipMemsetAsync in work a HIP graph may capture
+__global__ void k_fill_i32(int *p, int n, int v)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] = v;
+}
+__global__ void k_fill_u64(unsigned long long *p, long n)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) p[e] = 0ULL;
 }
 
 // rank map entries of band tiles set to -1 (rank_check.jdf on_band)

@@ -8,6 +8,7 @@ only marshals arguments; every computation runs in the HIP library, and
 loading fails loudly if the library is missing.
 """
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -33,7 +34,7 @@ EXPORTS = [
     "hicma_dense_logdet", "hicma_check_dpotrf2", "hicma_check_diff", "hicma_tlr_trsm", "hicma_tlr_potrs",
     "hicma_tlr_loglik", "hicma_dev_malloc", "hicma_dev_free", "hicma_memcpy", "hicma_tlr_loglik_noalloc",
     "hicma_init_ex", "hicma_server_create", "hicma_server_submit", "hicma_server_wait", "hicma_server_destroy",
-    "hicma_normal",
+    "hicma_normal", "hicma_factor_graph",
 ]
 
 _lib = None
@@ -121,6 +122,7 @@ def lib():
             "hicma_server_wait": (i, [vp, C.c_uint64, C.POINTER(LikResult)]),
             "hicma_server_destroy": (i, [vp]),
             "hicma_normal": (i, [u64, i, vp]),
+            "hicma_factor_graph": (i, [vp, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -198,10 +200,19 @@ class Server:
 class Context:
     def __init__(self, device=0):
         self.h = C.c_void_p()
+        self._owned = weakref.WeakSet()      # live objects holding device memory of this context
         _check(lib().hicma_init(C.byref(self.h), device), "hicma_init")
 
+    def _own(self, obj):
+        self._owned.add(obj)
+
     def close(self):
+        """Frees every object still open on this context, then the context
+        (an object kept alive past its context, e.g. by a traceback, must not
+        free memory through a context that no longer exists)."""
         if self.h:
+            for obj in list(self._owned):
+                obj.close()
             lib().hicma_fini(self.h)
             self.h = C.c_void_p()
 
@@ -228,6 +239,7 @@ class DeviceArray:
         self.host = np.array(host, dtype=np.float64, order="F", copy=True)
         self.p = C.c_void_p()
         _check(lib().hicma_dev_malloc(ctx.h, self.host.nbytes, C.byref(self.p)), "hicma_dev_malloc")
+        ctx._own(self)
         _check(lib().hicma_memcpy(ctx.h, self.p, self.host.ctypes.data, self.host.nbytes, 1), "hicma_memcpy")
 
     def get(self):
@@ -269,6 +281,7 @@ class Dense:
         self.ctx, self.N = ctx, N
         self.h = C.c_void_p()
         _check(lib().hicma_dense_create(ctx.h, C.byref(self.h), N), "hicma_dense_create")
+        ctx._own(self)
 
     def close(self):
         if self.h:
@@ -342,6 +355,7 @@ class TLRMatrix:
         self.h = C.c_void_p()
         _check(lib().hicma_desc_create(ctx.h, C.byref(self.h), N, NB, maxrank, kind, sigma2, beta, nugget, seed),
                "hicma_desc_create")
+        ctx._own(self)
         self.band = 1
         self.stats = None
 
@@ -445,6 +459,10 @@ class TLRMatrix:
                "hicma_tlr_loglik")
         X.close()
         return ld.value, qd.value, ll.value
+
+    def use_graph(self, enable=True):
+        """Capture the factorization in a HIP graph once, replay it after."""
+        _check(lib().hicma_factor_graph(self.h, int(enable)), "hicma_factor_graph")
 
     def jacobi_unconverged(self):
         return lib().hicma_jacobi_unconverged(self.h)

@@ -129,6 +129,15 @@ int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr_desc *
                    int compmaxrank, int send_full_tile, int *two_flow, unsigned long *tileopcounters,
                    unsigned long *opcounters, double *critical_path_time, int verbose);
 
+/* HiCMA_dpotrf_L on this descriptor captures its launch sequence (which
+ * depends only on NT and the band, not on the ranks: kernels read them from
+ * HBM) in a HIP graph on the first call and replays it on later calls while
+ * the descriptor's buffers, workspace and scalar arguments are unchanged,
+ * instead of the reference's parsec_context_start / wait
+ * (hicma_parsec/hicma_parsec.c:551-558, 595-602).  Calls that ask for
+ * critical-path times, rank history or profiling run without the graph. */
+int hicma_factor_graph(hicma_tlr_desc *A, int enable);
+
 /* op_counts (hicma_parsec/op_counts.h:10-72), for callers that account work */
 unsigned long hicma_op_counts(char op, unsigned long a, unsigned long b, unsigned long c, unsigned long d);
 /* 4 totals of the last factorization: band, offband, critical path, off path */

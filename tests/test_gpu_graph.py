@@ -44,11 +44,11 @@ def test_graph_replays_equal_eager(ctx, auto):
         lik = ll
     # a changed argument (fixed-rank mode) is captured anew, then eager again
     g.copy_from(pristine)
-    assert g.potrf(EPS, 20, U) == 0
+    assert g.potrf(EPS, 40, U) == 0
     fixed = state(g)
     g.use_graph(False)
     g.copy_from(pristine)
-    assert g.potrf(EPS, 20, U) == 0
+    assert g.potrf(EPS, 40, U) == 0
     eager_fixed = state(g)
     np.testing.assert_array_equal(fixed[1], eager_fixed[1])
     for x in (g, pristine):
