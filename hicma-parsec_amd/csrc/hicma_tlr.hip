@@ -554,7 +554,7 @@ struct PotrfWs {
     RcWs g;
     int ns, nsf, wgrid, wres, wcap;
     bool fast, wave;
-    size_t rc_lds, fstride, winv_d, tasks_d, wws_d, wbs_d, wlist_d, lrd_pairs, syrk_d, lrd_d, total;
+    size_t rc_lds, fstride, winv_d, dag_d, linv_d, tasks_d, wws_d, wbs_d, wlist_d, lrd_pairs, syrk_d, lrd_d, total;
 };
 
 static void plan_potrf_ws(hicma_ctx *ctx, int NT, int NB, int cap, int b, PotrfWs &P)
@@ -580,6 +580,11 @@ static void plan_potrf_ws(hicma_ctx *ctx, int NT, int NB, int cap, int b, PotrfW
     if (P.nsf > ctx->ncu) P.nsf = ctx->ncu;
     P.tasks_d = ((size_t)(maxtask > 0 ? maxtask : 1) + 1) / 2 + 1;  // ints packed in doubles
     P.winv_d = (size_t)((NB + PBK - 1) / PBK) * PBK * PBK;
+    // the diagonal potrf's task counter and done flags (ints packed in
+    // doubles) and the blocks of L_kk^-1
+    const size_t nbk = (size_t)((NB + PBK - 1) / PBK);
+    P.dag_d = (nbk * nbk + 2 + 1) / 2 + 1;
+    P.linv_d = nbk * nbk * PBK * PBK;
     // one-wave recompression (tasks with s <= WV_SMAX): the s <= 32 and
     // s <= 48 variants run 4 waves per workgroup, s <= 64 two
     P.wave = (NB <= 64 * WV_KMAX);
@@ -594,7 +599,7 @@ static void plan_potrf_ws(hicma_ctx *ctx, int NT, int NB, int cap, int b, PotrfW
     P.wbs_d = P.wave ? (size_t)P.wgrid * wb_slot_doubles(NB) : 0;  // one wide-stack wave per CU
     P.wlist_d = ((size_t)4 * P.wcap + 8) / 2 + 1;
     P.total = g.stride * P.ns + P.syrk_d + P.lrd_d + P.fstride * P.nsf + P.tasks_d + P.winv_d + P.wws_d +
-              P.wlist_d + P.wbs_d;
+              P.wlist_d + P.wbs_d + P.dag_d + P.linv_d;
 }
 
 // Device bytes a generation + factorization of this shape allocates on the
@@ -623,22 +628,40 @@ size_t hicma_desc_bytes(int N, int NB, int cap, int band, int ndim)
 // blocked Cholesky of diagonal tile k: per 64-column block, factor + invert
 // the diagonal block in LDS, panel = C W^T, lower trailing update.  The block
 // inverses (Winv, NB/64 blocks of 64x64) are reused by the low-rank trsm.
-static int launch_potrf(hicma_tlr_desc *A, int k, double *Winv, OpLog o, hipStream_t s)
+static int launch_potrf(hicma_tlr_desc *A, int k, double *Winv, double *Linv, int *dflags, OpLog o, hipStream_t s)
 {
     TLR &t = A->t;
-    for (int j0 = 0; j0 < t.NB; j0 += PBK) {
-        int jb = std::min(PBK, t.NB - j0);
-        hipLaunchKernelGGL(k_potrf_diag3, dim3(1), dim3(64), 0, s, t, k, j0, jb, Winv, A->ctx->d_info, o);
-        int R = t.NB - j0 - jb;
-        if (R > 0) {
-            hipLaunchKernelGGL(k_potrf_panel2, dim3((R + PBK - 1) / PBK), dim3(BS), 0, s, t, k, j0, jb,
-                               (const double *)Winv);
-            int nt = (R + GT - 1) / GT;
-            hipLaunchKernelGGL(k_potrf_update, dim3(nt * (nt + 1) / 2), dim3(BS), 0, s, t, k, j0, jb);
-        }
-    }
+    const int nb = (t.NB + PBK - 1) / PBK;
+    hipLaunchKernelGGL(k_fill_i32, dim3((nb * nb + 2 + 255) / 256), dim3(256), 0, s, dflags, nb * nb + 2, 0);
+    // one workgroup per block column: a whole column of tasks can run at once
+    hipLaunchKernelGGL(k_potrf_dag, dim3(nb), dim3(BS), sizeof(double) * DAG_SMEM, s, t, k, Winv, Linv, dflags,
+                       A->ctx->d_info, o);
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+// columns of V per workgroup of the low-rank trsm: its LDS holds TC columns
+// of 64 nb + 1 doubles
+static int trsm_tc(int NB)
+{
+    const size_t rld = (size_t)PBK * ((NB + PBK - 1) / PBK) + 1;
+    for (int tc = 16; tc >= 2; tc /= 2)
+        if (rld * tc * sizeof(double) <= TRSM_LDS_MAX) return tc;
+    return 0;
+}
+
+static void launch_trsm_lr(TLR &t, int k, int nlr, int cap, const double *Winv, const double *Linv, OpLog o,
+                           hipStream_t s)
+{
+    const int tc = trsm_tc(t.NB);
+    const size_t lds = sizeof(double) * ((size_t)PBK * ((t.NB + PBK - 1) / PBK) + 1) * tc;
+    const dim3 grid((cap + tc - 1) / tc, nlr);
+    switch (tc) {
+    case 16: hipLaunchKernelGGL(k_trsm_lr_inv<16>, grid, dim3(BS), lds, s, t, k, Winv, Linv, o); break;
+    case 8: hipLaunchKernelGGL(k_trsm_lr_inv<8>, grid, dim3(BS), lds, s, t, k, Winv, Linv, o); break;
+    case 4: hipLaunchKernelGGL(k_trsm_lr_inv<4>, grid, dim3(BS), lds, s, t, k, Winv, Linv, o); break;
+    default: hipLaunchKernelGGL(k_trsm_lr_inv<2>, grid, dim3(BS), lds, s, t, k, Winv, Linv, o); break;
+    }
 }
 
 extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr_desc *Ar, hicma_tlr_desc *Rank,
@@ -697,6 +720,19 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     int *wcount = (int *)(wws + wws_d);
     int *wlist = wcount + 8;  // counts [0..3], next-task counters [4..7]
     double *wbs = (double *)wcount + wlist_d;
+    double *Linv = wbs + P.wbs_d;
+    int *dflags = (int *)(Linv + P.linv_d);
+    HIPCHK(hipFuncSetAttribute((const void *)k_potrf_dag, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(sizeof(double) * DAG_SMEM)));
+    if (!trsm_tc(NB)) return HICMA_ERR_ARG;
+    HIPCHK(hipFuncSetAttribute((const void *)k_trsm_lr_inv<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)TRSM_LDS_MAX));
+    HIPCHK(hipFuncSetAttribute((const void *)k_trsm_lr_inv<8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)TRSM_LDS_MAX));
+    HIPCHK(hipFuncSetAttribute((const void *)k_trsm_lr_inv<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)TRSM_LDS_MAX));
+    HIPCHK(hipFuncSetAttribute((const void *)k_trsm_lr_inv<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)TRSM_LDS_MAX));
     if (fast)
         HIPCHK(hipFuncSetAttribute((const void *)k_lr_gemm_fast, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)(RCF_LDS * sizeof(double))));
@@ -759,6 +795,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     key.NT = NT;
     key.cap = cap;
     key.kind = t.kind;
+    key.wave_min_tasks = ctx->wave_min_tasks;
     const bool replay = use_graph && A->gexec && memcmp(&key, &A->gkey, sizeof(key)) == 0;
     if (use_graph && !replay) {
         if (A->gexec) (void)hipGraphExecDestroy(A->gexec);
@@ -782,7 +819,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     hipStream_t sd = ctx->side[2];
     if (NT > 0) {
         HIPCHK(cp(0, 0, st));
-        if ((rc = launch_potrf(A, 0, Winv, o, st))) return rc;
+        if ((rc = launch_potrf(A, 0, Winv, Linv, dflags, o, st))) return rc;
         HIPCHK(cp(0, 1, st));
     }
     for (int k = 0; k < NT; k++) {
@@ -792,7 +829,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
         if (nbnd > 0)
             hipLaunchKernelGGL(k_trsm_band, dim3(ntg, nbnd), dim3(BS), 0, st, t, k, o);
         if (nlr > 0)
-            hipLaunchKernelGGL(k_trsm_lr3, dim3((cap + TRSM_TC - 1) / TRSM_TC, nlr), dim3(BS), 0, st, t, k, (const double *)Winv, o);
+            launch_trsm_lr(t, k, nlr, cap, Winv, Linv, o, st);
         HIPCHK(cp(k, 3, st));
         HIPCHK(hipEventRecord(ctx->ev_join[2], st));
         HIPCHK(hipStreamWaitEvent(sd, ctx->ev_join[2], 0));
@@ -805,7 +842,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
         HIPCHK(cp(k, 5, sd));
         if (k + 1 < NT) {
             HIPCHK(cp(k + 1, 0, sd));
-            if ((rc = launch_potrf(A, k + 1, Winv, od, sd))) return rc;
+            if ((rc = launch_potrf(A, k + 1, Winv, Linv, dflags, od, sd))) return rc;
             HIPCHK(cp(k + 1, 1, sd));
         }
         if (nbnd >= 2)
@@ -829,8 +866,9 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
             hipLaunchKernelGGL(k_classify, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k, tasks_s);
             if (wave) {
                 hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, wcount, 8, 0);
+                const int wide_only = (fast && ntask < ctx->wave_min_tasks) ? 1 : 0;
                 hipLaunchKernelGGL(k_classify_wave, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k,
-                                   tasks_s, wlist, wcount, wcap);
+                                   tasks_s, wlist, wcount, wcap, wide_only);
                 // fork: variants 32 and 48 on the context stream, 64 and the
                 // wide one on side streams 0 and 1
                 HIPCHK(hipEventRecord(ctx->ev_fork, st));
@@ -994,6 +1032,13 @@ extern "C" int hicma_desc_copy(hicma_tlr_desc *dst, const hicma_tlr_desc *src)
 extern "C" int hicma_rank_overflows(const hicma_tlr_desc *A) { return A ? A->last_ovf : 0; }
 
 extern "C" int hicma_jacobi_unconverged(const hicma_tlr_desc *A) { return A ? A->last_unconv : 0; }
+
+extern "C" int hicma_set_wave_min_tasks(hicma_ctx *ctx, int ntask)
+{
+    if (!ctx || ntask < 0) return HICMA_ERR_ARG;
+    ctx->wave_min_tasks = ntask;
+    return 0;
+}
 
 extern "C" int hicma_factor_graph(hicma_tlr_desc *A, int enable)
 {

@@ -37,6 +37,10 @@ struct hicma_ctx {
     int *d_ovf;
     double *d_scalar;
     size_t lds_max;
+    // schedule: panel steps with fewer low-rank gemm tasks than this send
+    // the stacks of up to 60 columns to the 1024-thread recompression (one
+    // task per workgroup) instead of the one-wave kernels (0: never)
+    int wave_min_tasks;
     int profile;                     // record events around recompression launches
     std::vector<hipEvent_t> prof_ev; // pairs
     // the recompression variants of one panel step run concurrently on side
@@ -56,7 +60,7 @@ struct hicma_ctx {
 struct GraphKey {
     const void *band, *lr, *rank, *xyz, *ws, *tileops, *opstat;
     double acc, sigma2, beta, nugget;
-    int fixed_rank, maxrank, band_size, N, NB, NT, cap, kind;
+    int fixed_rank, maxrank, band_size, N, NB, NT, cap, kind, wave_min_tasks;
 };
 
 struct hicma_tlr_desc {

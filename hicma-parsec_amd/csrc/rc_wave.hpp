@@ -378,8 +378,9 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
 // appended to list 0 (s, rb <= 32), 1 (<= 48), 2 (s <= WV_S2MAX) or 3 (s <= 128,
 // rb <= WV_SMAX: k_rc_wide); the order inside
 // a list is irrelevant (tasks of one step are independent).  Listed tasks are
-// marked -1 in tasks_s; every other task keeps its k_classify value.
-__global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wcount, int cap)
+// marked -1 in tasks_s; every other task keeps its k_classify value.  With
+// wide_only, only list 3 is filled (the 1024-thread kernel takes the rest).
+__global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wcount, int cap, int wide_only)
 {
     const int T = t.NT - 1 - k;
     const long long ntask = (long long)T * (T - 1) / 2;
@@ -392,8 +393,9 @@ __global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wco
     const int ra = *rank_at(t, m, k), rc = *rank_at(t, m, n), rb = *rank_at(t, n, k);
     const int s = rc + ra;
     if (ra > 0 && rb > 0 && s <= 128 && rb <= WV_SMAX) {
-        tasks_s[e] = -1;
         const int l = (s <= 32 && rb <= 32) ? 0 : ((s <= 48 && rb <= 48) ? 1 : (s <= WV_S2MAX ? 2 : 3));
+        if (wide_only && l < 3) return;
+        tasks_s[e] = -1;
         wlist[(size_t)l * cap + atomicAdd(wcount + l, 1)] = (int)e;
     }
 }

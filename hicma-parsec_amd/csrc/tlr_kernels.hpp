@@ -2,6 +2,7 @@
 // (generation, dense and low-rank potrf / trsm / syrk / gemm, utilities).
 #pragma once
 #include "blk.hpp"
+#include "mfma.hpp"
 #include "tlr.hpp"
 
 // ======================= generation =======================
@@ -148,51 +149,26 @@ __global__ void __launch_bounds__(BS) k_compress(TLR t, const double *Om, double
 
 // ======================= dense potrf of a diagonal tile =======================
 
-// trailing lower update S -= P P^T, grid = lower GT tiles of the trailing block
-__global__ void __launch_bounds__(BS) k_potrf_update(TLR t, int k, int j0, int jb)
-{
-    __shared__ double sm[BLK_GEMM_SMEM];
-    const int NB = t.NB, j1 = j0 + jb, R = NB - j1;
-    int ti, tj;
-    tri_decode((long long)blockIdx.x, ti, tj);  // ti > tj
-    // map: linear over tiles with ti >= tj; use tri_decode on (ti+1)
-    ti -= 1;
-    double *A = band_tile(t, k, k);
-    const double *P = A + j1 + (size_t)j0 * NB;
-    double *S = A + j1 + (size_t)j1 * NB;
-    gemm_tile<false, true>(ti * GT, tj * GT, R, R, jb, -1.0, P, NB, P, NB, 1.0, S, NB, sm, true);
-}
-
 #define PBK 64
-#define PLD (PBK + 1)
-
-// Factors the jb x jb diagonal block at A(j0, j0) of tile (k, k) with one
-// wave, writes L back and W = L^{-1} (ld PBK) to Winv[j0 / PBK]: right-looking
-// Cholesky with lane i owning row i in registers (pivots by readlane, each
-// column of L broadcast through LDS), then W column by column with lane c
-// owning column c.
 #define PLR (PBK + 2)
-__global__ void __launch_bounds__(64) k_potrf_diag3(TLR t, int k, int j0, int jb, double *Winv, int *info, OpLog o)
+
+__device__ __forceinline__ void wave_lds_sync()
 {
-    if (j0 == 0 && threadIdx.x == 0) oplog(o, t.NT, opc('c', tdim(t, k), 0, 0, 0), k, k, true, true);
-    __shared__ double Lr[PBK * PLR];  // Lr[i * PLR + c] = L(i, c)
-    __shared__ double Wc[PBK * PLR];  // Wc[c * PLR + i] = W(i, c)
-    const int NB = t.NB, lane = threadIdx.x;
-    double *A = band_tile(t, k, k) + j0 + (size_t)j0 * NB;
-    {
-        // all 64 loads in flight before the LDS stores
-        double row[PBK];
-#pragma unroll
-        for (int c = 0; c < PBK; c++) row[c] = (c < jb && lane < jb && lane >= c) ? A[lane + (size_t)c * NB] : 0.0;
-#pragma unroll
-        for (int c = 0; c < PBK; c++) Lr[lane * PLR + c] = row[c];
-    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // right-looking Cholesky, lane = row: a[c] = A(lane, c) is updated in
-    // registers; column j of L is broadcast through LDS (Wc[0 .. 63] as a
-    // buffer), so each step issues up to 63 independent FMAs
+}
+
+// Factors the jb x jb block held (lower part, row major) in Lr[i * PLR + c]
+// with ONE wave and forms W = L^-1: right-looking Cholesky with lane i owning
+// row i in registers (pivots by readlane, each column of L broadcast through
+// LDS, so each step issues up to 63 independent FMAs), then W column by
+// column with lane c owning column c.  On return Lr holds L (zero above the
+// diagonal and outside jb) and Wc[c * PLR + i] = W(i, c).  Returns the
+// 1-based index of the first non-positive pivot, or 0.
+__device__ int diag_factor_wave(double *Lr, double *Wc, int jb)
+{
+    const int lane = threadIdx.x & 63;
     double a[PBK];
 #pragma unroll
     for (int c = 0; c < PBK; c++) a[c] = Lr[lane * PLR + c];
@@ -202,194 +178,243 @@ __global__ void __launch_bounds__(64) k_potrf_diag3(TLR t, int k, int j0, int jb
         if (j < jb) {
             const double d2 = mk64((unsigned)__builtin_amdgcn_readlane((int)lo32(a[j]), j),
                                    (unsigned)__builtin_amdgcn_readlane((int)hi32(a[j]), j));
-            if (!(d2 > 0.0)) {
-                if (lane == 0) atomicCAS(info, 0, k * NB + j0 + j + 1);
-                return;
-            }
+            if (!(d2 > 0.0)) return j + 1;
             const double d = sqrt(d2);
             const double lij = (lane == j) ? d : ((lane > j && lane < jb) ? a[j] / d : 0.0);
             a[j] = lij;
             colj[lane] = lij;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            wave_lds_sync();
 #pragma unroll
             for (int c = j + 1; c < PBK; c++) a[c] = fma(-lij, colj[c], a[c]);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            wave_lds_sync();
         }
     }
     // L (lower part of the rows) back to LDS
 #pragma unroll
     for (int c = 0; c < PBK; c++) Lr[lane * PLR + c] = (c <= lane && lane < jb) ? a[c] : 0.0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_lds_sync();
     // W = L^-1, right-looking, lane = column c: w(l) = x(l) / L(l, l), then
     // x(i) -= L(i, l) w(l) for i > l (L entries are broadcast LDS reads)
-    {
 #pragma unroll
-        for (int i = 0; i < PBK; i++) a[i] = (i == lane) ? 1.0 : 0.0;
+    for (int i = 0; i < PBK; i++) a[i] = (i == lane) ? 1.0 : 0.0;
 #pragma unroll
-        for (int l = 0; l < PBK; l++) {
-            if (l < jb) {
-                const double w = (lane <= l) ? a[l] / Lr[l * PLR + l] : 0.0;
-                a[l] = w;
+    for (int l = 0; l < PBK; l++) {
+        if (l < jb) {
+            const double w = (lane <= l) ? a[l] / Lr[l * PLR + l] : 0.0;
+            a[l] = w;
 #pragma unroll
-                for (int i = l + 1; i < PBK; i++) a[i] = fma(-Lr[i * PLR + l], w, a[i]);
+            for (int i = l + 1; i < PBK; i++) a[i] = fma(-Lr[i * PLR + l], w, a[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < PBK; i++) Wc[lane * PLR + i] = (i < jb && lane < jb) ? a[i] : 0.0;
+    wave_lds_sync();
+    return 0;
+}
+
+// ---- the diagonal tile's Cholesky and its inverse as one launch: a
+// device-side dataflow schedule over 64 x 64 blocks.
+//
+// Factor task (i, j), i >= j: S = A_ij - sum_{l<j} L_il L_jl^T (64-deep MFMA
+// chunks, chunk l as soon as blocks (i, l) and (j, l) are done); then i == j
+// factors S (L_jj and W_j = L_jj^-1, one wave) and i > j forms
+// L_ij = S W_j^T once (j, j) is done.
+// Inverse task (i, j), i > j: Linv_ij = -W_i sum_{l=j}^{i-1} L_il Linv_lj
+// (Linv_jj = W_j), the blocks of L^-1 that the low-rank trsm multiplies by.
+//
+// Numbering: block column c holds the factor tasks (c .. nb-1, c), then the
+// inverse tasks (c, 0 .. c-1); workgroups claim tasks in that order from an
+// atomic counter.  Every task waits only for tasks with smaller numbers,
+// which started earlier and wait for nothing larger, so the schedule cannot
+// deadlock; each wait is bounded all the same (a wait that runs out marks
+// flags[1] and the task goes on).
+//
+// flags: [0] task counter, [1] wait ran out, [2 + task] done; zero at launch.
+// Publishing a block: its stores, an agent-scope release fence in every
+// thread (stores complete, L2 written back: another XCD may read the block),
+// a barrier and the done flag; reading: the flag, an agent-scope acquire
+// fence (L1 and L2 invalidated), a barrier.
+#define DAG_SPIN_LIMIT (1 << 21)
+#define DAG_SMEM (2 * PBK * PLR + MK_SMEM)
+
+__device__ __forceinline__ int dag_fac(int i, int j, int nb) { return j * nb + (i - j); }
+__device__ __forceinline__ int dag_inv(int i, int j, int nb) { return i * nb + (nb - i) + j; }
+
+__device__ __forceinline__ void dag_wait2(int *flags, int t0, int t1)
+{
+    if (threadIdx.x == 0) {
+        int n = 0;
+        const int *f0 = flags + 2 + t0, *f1 = flags + 2 + t1;
+        while (!__hip_atomic_load(f0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+               !__hip_atomic_load(f1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++n > DAG_SPIN_LIMIT) {
+                __hip_atomic_store(flags + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
             }
         }
-#pragma unroll
-        for (int i = 0; i < PBK; i++) Wc[lane * PLR + i] = (i < jb && lane < jb) ? a[i] : 0.0;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-    double *Wo = Winv + (size_t)(j0 / PBK) * PBK * PBK;
-#pragma unroll 8
-    for (int c = 0; c < jb; c++) {
-        if (lane < jb && lane >= c) A[lane + (size_t)c * NB] = Lr[lane * PLR + c];
-        if (lane < jb) Wo[lane + c * PBK] = Wc[c * PLR + lane];
-    }
-}
-
-// panel rows below the diagonal block: X = C W^T (W = L^{-1}), in place;
-// grid = ceil(rows / PBK)
-__global__ void __launch_bounds__(BS) k_potrf_panel2(TLR t, int k, int j0, int jb, const double *Winv)
-{
-    __shared__ double Cs[PBK * PLD];
-    __shared__ double Ws[PBK * PLD];
-    const int NB = t.NB, tid = threadIdx.x;
-    const int r0 = j0 + jb + blockIdx.x * PBK;
-    int nr = NB - r0;
-    if (nr <= 0) return;
-    if (nr > PBK) nr = PBK;
-    double *A = band_tile(t, k, k);
-    const double *Wi = Winv + (size_t)(j0 / PBK) * PBK * PBK;
-    for (int e = tid; e < PBK * jb; e += BS) {
-        int i = e % PBK, c = e / PBK;
-        Cs[i + c * PLD] = (i < nr) ? A[r0 + i + (size_t)(j0 + c) * NB] : 0.0;
-        if (i < jb) Ws[i + c * PLD] = Wi[i + c * PBK];
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    // thread: row i = tid % 64, columns c = tid / 64 + 4 q
-    const int i = tid % PBK, cg = tid / PBK;
-    double x[PBK / (BS / PBK)];
-#pragma unroll
-    for (int q = 0; q < PBK / (BS / PBK); q++) {
-        const int c = cg + q * (BS / PBK);
-        double s = 0.0;
-        if (c < jb)
-            for (int l = 0; l <= c; l++) s = fma(Cs[i + l * PLD], Ws[c + l * PLD], s);
-        x[q] = s;
-    }
-#pragma unroll
-    for (int q = 0; q < PBK / (BS / PBK); q++) {
-        const int c = cg + q * (BS / PBK);
-        if (c < jb && i < nr) A[r0 + i + (size_t)(j0 + c) * NB] = x[q];
+}
+
+__device__ __forceinline__ void dag_done(int *flags, int task)
+{
+    // every thread's stores of the block complete and are written back from
+    // L2 before the barrier; then the flag
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(flags + 2 + task, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Linv: nb x nb blocks of 64 x 64 (column major, ld 64) at Linv + (i nb + j) 4096,
+// written for i > j; the diagonal blocks are Winv's
+__global__ void __launch_bounds__(BS) k_potrf_dag(TLR t, int k, double *Winv, double *Linv, int *flags, int *info,
+                                                 OpLog o)
+{
+    extern __shared__ double sm[];
+    double *Lr = sm, *Wc = sm + PBK * PLR, *Mk = sm + 2 * PBK * PLR;
+    __shared__ int s_task;
+    const int NB = t.NB, nb = (NB + PBK - 1) / PBK, ntask = nb * nb, tid = threadIdx.x;
+    double *A = band_tile(t, k, k);
+    for (;;) {
+        __syncthreads();  // s_task and the LDS buffers are free
+        if (tid == 0) s_task = atomicAdd(flags, 1);
+        __syncthreads();
+        const int task = s_task;
+        if (task >= ntask) break;
+        const int col = task / nb, pos = task % nb;
+        const bool inverse = pos >= nb - col;
+        const int i = inverse ? col : col + pos, j = inverse ? pos - (nb - col) : col;
+        const int r0 = PBK * i, c0 = PBK * j;
+        const int ri = min(PBK, NB - r0), cj = min(PBK, NB - c0);
+        if (task == 0 && tid == 0) oplog(o, t.NT, opc('c', tdim(t, k), 0, 0, 0), k, k, true, true);
+        mk_d4 acc[2][2];
+        mk_zero(acc);
+        if (inverse) {
+            // S = sum_{l=j}^{i-1} L_il Linv_lj, then Linv_ij = -W_i S
+            for (int l = j; l < i; l++) {
+                dag_wait2(flags, dag_fac(i, l, nb), l == j ? dag_fac(j, j, nb) : dag_inv(l, j, nb));
+                const double *blk = (l == j) ? Winv + (size_t)j * PBK * PBK : Linv + ((size_t)l * nb + j) * PBK * PBK;
+                const MkOps<false, true> op{A + r0, NB, blk - (long)PBK * l, PBK};
+                mk_accum(op, ri, cj, 0, 0, (long)PBK * l, (long)PBK * l + PBK, acc, Mk);
+            }
+            mk_store(acc, [&](int ii, int jj, double v) { Lr[ii * PLR + jj] = v; });  // S(ii, jj)
+            dag_wait2(flags, dag_fac(i, i, nb), dag_fac(i, i, nb));
+            const double *Wi = Winv + (size_t)i * PBK * PBK;
+            for (int e = tid; e < PBK * PBK; e += BS) {
+                const int ii = e % PBK, l = e / PBK;  // W_i(ii, l), column major
+                Wc[ii * PLR + l] = (ii < ri && l < ri) ? Wi[ii + l * PBK] : 0.0;
+            }
+            __syncthreads();
+            // Linv_ij(ii, c) = -sum_{l <= ii} W_i(ii, l) S(l, c); thread: row ii, columns cg + 4 q
+            const int ii = tid % PBK, cg = tid / PBK;
+            double *Lo = Linv + ((size_t)i * nb + j) * PBK * PBK;
+#pragma unroll 4
+            for (int q = 0; q < PBK / 4; q++) {
+                const int c = cg + 4 * q;
+                double x = 0.0;
+                for (int l = 0; l <= ii; l++) x = fma(Wc[ii * PLR + l], Lr[l * PLR + c], x);
+                Lo[ii + c * PBK] = -x;
+            }
+        } else {
+            // S = A_ij - sum_l L_il L_jl^T
+            const MkOps<false, false> op{A + r0, NB, A + c0, NB};
+            for (int l = 0; l < j; l++) {
+                dag_wait2(flags, dag_fac(i, l, nb), dag_fac(j, l, nb));
+                mk_accum(op, ri, cj, 0, 0, (long)PBK * l, (long)PBK * l + PBK, acc, Mk);
+            }
+            if (i == j) {
+                // S (lower) row major into Lr, factor and invert with wave 0
+                mk_store(acc, [&](int ii, int jj, double v) {
+                    Lr[ii * PLR + jj] = (ii < cj && jj <= ii) ? A[r0 + ii + (size_t)(c0 + jj) * NB] - v : 0.0;
+                });
+                __syncthreads();
+                if (tid < 64) {
+                    const int bad = diag_factor_wave(Lr, Wc, cj);
+                    if (bad) {
+                        if (tid == 0) atomicCAS(info, 0, k * NB + c0 + bad);
+                    } else {
+                        // the whole 64 x 64 W_j, zero outside cj x cj (the trsm
+                        // multiplies the padding by zero rows)
+                        double *Wo = Winv + (size_t)j * PBK * PBK;
+                        for (int c = 0; c < PBK; c++) {
+                            if (c < cj && tid < cj && tid >= c) A[r0 + tid + (size_t)(c0 + c) * NB] = Lr[tid * PLR + c];
+                            Wo[tid + c * PBK] = Wc[c * PLR + tid];
+                        }
+                    }
+                }
+            } else {
+                // S row major into Lr; W_j (written by task (j, j)) into Wc
+                mk_store(acc, [&](int ii, int jj, double v) {
+                    Lr[ii * PLR + jj] = (ii < ri && jj < cj) ? A[r0 + ii + (size_t)(c0 + jj) * NB] - v : 0.0;
+                });
+                dag_wait2(flags, dag_fac(j, j, nb), dag_fac(j, j, nb));
+                const double *Wj = Winv + (size_t)j * PBK * PBK;
+                for (int e = tid; e < PBK * PBK; e += BS) {
+                    const int ii = e % PBK, c = e / PBK;  // W(ii, c), column major
+                    Wc[c * PLR + ii] = (ii < cj && c < cj) ? Wj[ii + c * PBK] : 0.0;
+                }
+                __syncthreads();
+                // L_ij(ii, c) = sum_{l <= c} S(ii, l) W(c, l); thread: row ii, columns cg + 4 q
+                const int ii = tid % PBK, cg = tid / PBK;
+#pragma unroll 4
+                for (int q = 0; q < PBK / 4; q++) {
+                    const int c = cg + 4 * q;
+                    double x = 0.0;
+                    for (int l = 0; l <= c; l++) x = fma(Lr[ii * PLR + l], Wc[l * PLR + c], x);
+                    if (ii < ri && c < cj) A[r0 + ii + (size_t)(c0 + c) * NB] = x;
+                }
+            }
+        }
+        dag_done(flags, task);
     }
 }
 
-// low-rank tiles m = k+band .. NT-1: V <- L^{-1} V, TRSM_TC columns of one tile
-// per workgroup.  Row block b (64 rows) is X_b = W_b (V_b - L_b,0:b X_0:b): the
-// sum runs over 64-deep chunks of L and of the solved rows staged in LDS (so
-// every global load of a chunk is in flight at once), then the inverted
-// diagonal block W_b is applied from LDS.  grid (ceil(cap/TRSM_TC), count).
-#define TL3 (PBK + 1)
-#define TRSM_TC 8                   // columns of V per workgroup
-#define TRSM_XL (TRSM_TC + 1)       // LDS row stride of X
-__global__ void __launch_bounds__(BS) k_trsm_lr3(TLR t, int k, const double *Winv, OpLog o)
+// low-rank tiles m = k+band .. NT-1: V <- L^-1 V as one product with the
+// block inverse of L_kk (k_potrf_dag), TC columns of one tile per workgroup.
+// The columns are staged in LDS (rows padded to 64 nb, stride 64 nb + 1), so
+// the product can overwrite V in place; wave w computes rows 16 w .. 16 w + 15
+// of every 64-row block b on the matrix cores: sum over c <= b of
+// Linv_bc (16 x 64) V_c (64 x TC), the 16 Linv values of a 64-deep chunk
+// loaded before its MFMAs.  grid (ceil(cap / TC), count).
+#define TRSM_LDS_MAX (150 * 1024)  // bytes of LDS the staged columns may take
+template <int TC>
+__global__ void __launch_bounds__(BS) k_trsm_lr_inv(TLR t, int k, const double *Winv, const double *Linv, OpLog o)
 {
-    __shared__ double Ls[PBK * TL3];   // Ls[kk * TL3 + i] = L(b0 + i, l0 + kk); also W_b
-    __shared__ double Xs[PBK * TRSM_XL];  // Xs[kk * TRSM_XL + c] = X(l0 + kk, c); also V_b
-    const int NB = t.NB, tid = threadIdx.x;
+    extern __shared__ double Vl[];
+    const int NB = t.NB, nb = (NB + PBK - 1) / PBK, RLD = PBK * nb + 1, tid = threadIdx.x;
     const int m = k + t.band_size + blockIdx.y;
     const int r = *rank_at(t, m, k);
     if (blockIdx.x == 0 && tid == 0) oplog(o, t.NT, opc('t', tdim(t, m), r, 1, 0), m, k, false, m - k == 1);
-    const int c0 = blockIdx.x * TRSM_TC;
+    const int c0 = blockIdx.x * TC;
     if (c0 >= r) return;
-    const int nc = (r - c0) < TRSM_TC ? (r - c0) : TRSM_TC;
+    const int nc = (r - c0) < TC ? (r - c0) : TC;
     double *V = lr_tile(t, m, k) + (size_t)NB * r + (size_t)c0 * NB;
-    const double *L = band_tile(t, k, k);
-    constexpr int QC = TRSM_TC / (BS / PBK);  // columns per thread
-    const int i = tid % PBK, cg = tid / PBK;  // row i, columns QC cg .. QC cg + QC - 1
-    // software pipeline: the next chunk of L and X, and the block's W_b, are
-    // loaded into registers while the current chunk is multiplied from LDS
-    constexpr int LE = PBK * PBK / BS, XE = PBK * TRSM_TC / BS;
-    double pl[LE], px[XE], pw[LE];
-    auto load_chunk = [&](int b0, int nb, int l0) {
+    for (int e = tid; e < PBK * nb * TC; e += BS) {
+        const int row = e % (PBK * nb), c = e / (PBK * nb);
+        Vl[row + c * RLD] = (row < NB && c < nc) ? V[row + (size_t)c * NB] : 0.0;
+    }
+    __syncthreads();
+    const int lane = tid & 63, w = tid >> 6, i16 = lane & 15, k4 = lane >> 4;
+    const bool colok = i16 < TC;
+    for (int b = 0; b < nb; b++) {
+        mk_d4 acc = mk_d4{0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c <= b; c++) {
+            const double *blk = (c == b) ? Winv + (size_t)b * PBK * PBK : Linv + ((size_t)b * nb + c) * PBK * PBK;
+            const double *arow = blk + 16 * w + i16;
+            double a[16];
 #pragma unroll
-        for (int j = 0; j < LE; j++) {
-            const int e = tid + BS * j, ii = e % PBK, kk = e / PBK;
-            pl[j] = (ii < nb) ? L[b0 + ii + (size_t)(l0 + kk) * NB] : 0.0;
+            for (int q = 0; q < 16; q++) a[q] = arow[(4 * q + k4) * PBK];
+            const double *vcol = Vl + PBK * c + k4 + i16 * RLD;
+#pragma unroll
+            for (int q = 0; q < 16; q++) acc = mk_mfma(a[q], colok ? vcol[4 * q] : 0.0, acc);
         }
 #pragma unroll
-        for (int j = 0; j < XE; j++) {
-            const int e = tid + BS * j, kk = e % PBK, c = e / PBK;
-            px[j] = (c < nc) ? V[l0 + kk + (size_t)c * NB] : 0.0;
+        for (int rr = 0; rr < 4; rr++) {
+            const int row = PBK * b + 16 * w + k4 + 4 * rr;
+            if (row < NB && i16 < nc) V[row + (size_t)i16 * NB] = acc[rr];
         }
-    };
-    for (int b0 = 0; b0 < NB; b0 += PBK) {
-        const int nb = (NB - b0) < PBK ? (NB - b0) : PBK;
-        const double *Wb = Winv + (size_t)(b0 / PBK) * PBK * PBK;
-#pragma unroll
-        for (int j = 0; j < LE; j++) {
-            const int e = tid + BS * j, ii = e % PBK, l = e / PBK;
-            pw[j] = Wb[ii + (size_t)l * PBK];
-        }
-        double acc[QC];
-#pragma unroll
-        for (int q = 0; q < QC; q++) {
-            const int c = cg * QC + q;
-            acc[q] = (i < nb && c < nc) ? V[b0 + i + (size_t)c * NB] : 0.0;
-        }
-        if (b0 > 0) load_chunk(b0, nb, 0);
-        for (int l0 = 0; l0 < b0; l0 += PBK) {
-            // L(b0:b0+64, l0:l0+64) and X(l0:l0+64, 0:TRSM_TC) from registers to LDS
-#pragma unroll
-            for (int j = 0; j < LE; j++) {
-                const int e = tid + BS * j, ii = e % PBK, kk = e / PBK;
-                Ls[kk * TL3 + ii] = pl[j];
-            }
-#pragma unroll
-            for (int j = 0; j < XE; j++) {
-                const int e = tid + BS * j, kk = e % PBK, c = e / PBK;
-                Xs[kk * TRSM_XL + c] = px[j];
-            }
-            __syncthreads();
-            if (l0 + PBK < b0) load_chunk(b0, nb, l0 + PBK);
-#pragma unroll 8
-            for (int kk = 0; kk < PBK; kk++) {
-                const double lv = Ls[kk * TL3 + i];
-#pragma unroll
-                for (int q = 0; q < QC; q++) acc[q] = fma(-lv, Xs[kk * TRSM_XL + cg * QC + q], acc[q]);
-            }
-            __syncthreads();
-        }
-        // X_b = W_b acc: W_b (lower triangular 64 x 64) into Ls, acc into Xs
-#pragma unroll
-        for (int j = 0; j < LE; j++) {
-            const int e = tid + BS * j, ii = e % PBK, l = e / PBK;
-            Ls[l * TL3 + ii] = pw[j];
-        }
-#pragma unroll
-        for (int q = 0; q < QC; q++) Xs[i * TRSM_XL + cg * QC + q] = acc[q];
-        __syncthreads();
-        double x[QC];
-#pragma unroll
-        for (int q = 0; q < QC; q++) x[q] = 0.0;
-        for (int l = 0; l <= i && l < nb; l++) {
-            const double w = Ls[l * TL3 + i];
-#pragma unroll
-            for (int q = 0; q < QC; q++) x[q] = fma(w, Xs[l * TRSM_XL + cg * QC + q], x[q]);
-        }
-#pragma unroll
-        for (int q = 0; q < QC; q++) {
-            const int c = cg * QC + q;
-            if (i < nb && c < nc) V[b0 + i + (size_t)c * NB] = x[q];
-        }
-        __syncthreads();
     }
 }
 

@@ -34,7 +34,7 @@ EXPORTS = [
     "hicma_dense_logdet", "hicma_check_dpotrf2", "hicma_check_diff", "hicma_tlr_trsm", "hicma_tlr_potrs",
     "hicma_tlr_loglik", "hicma_dev_malloc", "hicma_dev_free", "hicma_memcpy", "hicma_tlr_loglik_noalloc",
     "hicma_init_ex", "hicma_server_create", "hicma_server_submit", "hicma_server_wait", "hicma_server_destroy",
-    "hicma_normal", "hicma_factor_graph",
+    "hicma_normal", "hicma_factor_graph", "hicma_set_wave_min_tasks",
 ]
 
 _lib = None
@@ -123,6 +123,7 @@ def lib():
             "hicma_server_destroy": (i, [vp]),
             "hicma_normal": (i, [u64, i, vp]),
             "hicma_factor_graph": (i, [vp, i]),
+            "hicma_set_wave_min_tasks": (i, [vp, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -218,6 +219,9 @@ class Context:
 
     def profile(self, enable=True):
         _check(lib().hicma_profile(self.h, int(enable)), "hicma_profile")
+
+    def set_wave_min_tasks(self, ntask):
+        _check(lib().hicma_set_wave_min_tasks(self.h, int(ntask)), "hicma_set_wave_min_tasks")
 
     def profile_read(self):
         ms, n = C.c_double(), C.c_int()

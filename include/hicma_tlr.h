@@ -138,6 +138,15 @@ int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr_desc *
  * critical-path times, rank history or profiling run without the graph. */
 int hicma_factor_graph(hicma_tlr_desc *A, int enable);
 
+/* Schedule of the low-rank gemm + recompression tasks: panel steps with
+ * fewer tasks than ntask give the stacks of up to 60 columns one 1024-thread
+ * workgroup each instead of one wave (fewer tasks, shorter latency per task);
+ * 0 (the default) keeps every step on the one-wave kernels.  No counterpart
+ * in the reference (its PaRSEC runtime schedules tasks on CPU cores).
+ * Results depend on the schedule only through rounding, and are the same for
+ * the same schedule. */
+int hicma_set_wave_min_tasks(hicma_ctx *ctx, int ntask);
+
 /* op_counts (hicma_parsec/op_counts.h:10-72), for callers that account work */
 unsigned long hicma_op_counts(char op, unsigned long a, unsigned long b, unsigned long c, unsigned long d);
 /* 4 totals of the last factorization: band, offband, critical path, off path */
