@@ -5,6 +5,7 @@
 // Matrices are column major.  LDS scratch is passed in explicitly so a
 // kernel can overlay the scratch of successive phases.
 #pragma once
+#include "mfma.hpp"
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "wave.hpp"
@@ -93,69 +94,27 @@ __device__ __forceinline__ void blk_vsum(const double *p, int n, double *red, do
     __syncthreads();
 }
 
-// One GT x GT output tile of C = alpha*op(A)*op(B) + beta*C at (i0, j0).
-// 16x16 threads, each owning a 4x4 register block (rows tx+16i, cols ty+16j).
-// lower_only: skip elements strictly above the diagonal of C.
+// One GT x GT output tile of C = alpha*op(A)*op(B) + beta*C at (i0, j0) on
+// the FP64 matrix cores (mk_accum: 4 waves, 2 x 2 blocks of 16 x 16 each,
+// K in LDS-staged chunks).  lower_only: skip elements strictly above the
+// diagonal of C.  All BS threads call it.
 template <bool TA, bool TB>
 __device__ void gemm_tile(int i0, int j0, int M, int N, int K, double alpha,
                           const double *__restrict__ A, int lda,
                           const double *__restrict__ B, int ldb, double beta,
                           double *C, int ldc, double *sm, bool lower_only = false)
 {
-    double *As = sm, *Bs = sm + GK * GLD;
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    double acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
-    for (int k0 = 0; k0 < K; k0 += GK) {
-        for (int e = tid; e < GT * GK; e += BS) {
-            int ii, kk;
-            if (!TA) { ii = e & (GT - 1); kk = e >> 6; }
-            else { kk = e & (GK - 1); ii = e >> 4; }
-            int gi = i0 + ii, gk = k0 + kk;
-            double v = 0.0;
-            if (gi < M && gk < K) v = TA ? A[gk + (size_t)gi * lda] : A[gi + (size_t)gk * lda];
-            As[kk * GLD + ii] = v;
-        }
-        for (int e = tid; e < GT * GK; e += BS) {
-            int jj, kk;
-            if (!TB) { kk = e & (GK - 1); jj = e >> 4; }
-            else { jj = e & (GT - 1); kk = e >> 6; }
-            int gj = j0 + jj, gk = k0 + kk;
-            double v = 0.0;
-            if (gj < N && gk < K) v = TB ? B[gj + (size_t)gk * ldb] : B[gk + (size_t)gj * ldb];
-            Bs[kk * GLD + jj] = v;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < GK; kk++) {
-            double a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = As[kk * GLD + tx + 16 * i];
-#pragma unroll
-            for (int j = 0; j < 4; j++) b[j] = Bs[kk * GLD + ty + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        int gj = j0 + ty + 16 * j;
-        if (gj >= N) continue;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            int gi = i0 + tx + 16 * i;
-            if (gi >= M) continue;
-            if (lower_only && gi < gj) continue;
-            double *c = C + gi + (size_t)gj * ldc;
-            *c = (beta == 0.0) ? alpha * acc[i][j] : fma(alpha, acc[i][j], beta * (*c));
-        }
-    }
+    static_assert(GT == MK_T && BLK_GEMM_SMEM >= MK_SMEM && BS == 256, "gemm tile shape");
+    mk_d4 acc[2][2];
+    mk_zero(acc);
+    const MkOps<TA, !TB> op{A, lda, B, ldb};
+    mk_accum(op, M, N, i0, j0, 0, K, acc, sm);
+    mk_store(acc, [&](int ii, int jj, double v) {
+        const int gi = i0 + ii, gj = j0 + jj;
+        if (gi >= M || gj >= N || (lower_only && gi < gj)) return;
+        double *c = C + gi + (size_t)gj * ldc;
+        *c = (beta == 0.0) ? alpha * v : fma(alpha, v, beta * (*c));
+    });
 }
 
 // whole GEMM by one workgroup
