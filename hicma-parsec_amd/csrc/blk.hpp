@@ -96,8 +96,11 @@ __device__ __forceinline__ void blk_vsum(const double *p, int n, double *red, do
 
 // One GT x GT output tile of C = alpha*op(A)*op(B) + beta*C at (i0, j0) on
 // the FP64 matrix cores (mk_accum: 4 waves, 2 x 2 blocks of 16 x 16 each,
-// K in LDS-staged chunks).  lower_only: skip elements strictly above the
-// diagonal of C.  All BS threads call it.
+// K in LDS-staged chunks).  The tile is computed as C^T = op(B)^T op(A)^T:
+// the instruction's result layout then puts consecutive rows of C on
+// consecutive lanes, so the column-major read-modify-write of C coalesces.
+// lower_only: skip elements strictly above the diagonal of C.  All BS
+// threads call it.
 template <bool TA, bool TB>
 __device__ void gemm_tile(int i0, int j0, int M, int N, int K, double alpha,
                           const double *__restrict__ A, int lda,
@@ -107,9 +110,10 @@ __device__ void gemm_tile(int i0, int j0, int M, int N, int K, double alpha,
     static_assert(GT == MK_T && BLK_GEMM_SMEM >= MK_SMEM && BS == 256, "gemm tile shape");
     mk_d4 acc[2][2];
     mk_zero(acc);
-    const MkOps<TA, !TB> op{A, lda, B, ldb};
-    mk_accum(op, M, N, i0, j0, 0, K, acc, sm);
-    mk_store(acc, [&](int ii, int jj, double v) {
+    // operand 1 (j, k) = op(B)(k, j), operand 2 (k, i) = op(A)(i, k)
+    const MkOps<!TB, TA> op{B, ldb, A, lda};
+    mk_accum(op, N, M, j0, i0, 0, K, acc, sm);
+    mk_store(acc, [&](int jj, int ii, double v) {
         const int gi = i0 + ii, gj = j0 + jj;
         if (gi >= M || gj >= N || (lower_only && gi < gj)) return;
         double *c = C + gi + (size_t)gj * ldc;
