@@ -12,6 +12,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <algorithm>
+#include <chrono>
 
 // ---------------------------------------------------------------------------
 // point set: jittered grid, Morton order (spec of the build; see DESIGN)
@@ -866,7 +867,7 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
             hipLaunchKernelGGL(k_classify, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k, tasks_s);
             if (wave) {
                 hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, wcount, 8, 0);
-                const int wide_only = (fast && ntask < ctx->wave_min_tasks) ? 1 : 0;
+                const int wide_only = (ntask < ctx->wave_min_tasks) ? 1 : 0;
                 hipLaunchKernelGGL(k_classify_wave, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k,
                                    tasks_s, wlist, wcount, wcap, wide_only);
                 // fork: variants 32 and 48 on the context stream, 64 and the
@@ -1037,6 +1038,49 @@ extern "C" int hicma_set_wave_min_tasks(hicma_ctx *ctx, int ntask)
 {
     if (!ctx || ntask < 0) return HICMA_ERR_ARG;
     ctx->wave_min_tasks = ntask;
+    return 0;
+}
+
+// Times HiCMA_dpotrf_L on a copy of A for each candidate schedule (steps
+// with fewer tasks than the threshold on the four-wave kernel) and keeps the
+// fastest in the context.  A is not modified.
+extern "C" int hicma_autotune_schedule(hicma_ctx *ctx, hicma_tlr_desc *A, double acc, int maxrank, int reps,
+                                       int *best, double *best_seconds)
+{
+    if (!ctx || !A || A->ctx != ctx || reps < 1) return HICMA_ERR_ARG;
+    const TLR &t = A->t;
+    const long long maxtask = (long long)(t.NT - 1) * (t.NT - 2) / 2;
+    std::vector<int> cand{0};
+    for (int c = 50; c < 4 * maxtask && c <= 6400; c *= 2) cand.push_back(c);
+    hicma_tlr_desc *W = nullptr;
+    RCHK(hicma_desc_create(ctx, &W, t.N, t.NB, t.cap, t.kind, t.sigma2, t.beta, t.nugget, A->seed));
+    const int saved = ctx->wave_min_tasks;
+    int bi = saved, rc = 0;
+    double bs = 1e300;
+    for (int c : cand) {
+        ctx->wave_min_tasks = c;
+        double tmin = 1e300;
+        for (int r = 0; r < reps && !rc; r++) {
+            if ((rc = hicma_desc_copy(W, A))) break;
+            int la = -1, tf = 0;
+            const auto t0 = std::chrono::steady_clock::now();
+            const int info = HiCMA_dpotrf_L(ctx, HICMA_LOWER, W, W, W, acc, 0, maxrank, &la, t.band_size, 300, maxrank,
+                                            0, &tf, NULL, NULL, NULL, 0);
+            const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (info) rc = info;
+            tmin = std::min(tmin, sec);
+        }
+        if (rc) break;
+        if (tmin < bs) {
+            bs = tmin;
+            bi = c;
+        }
+    }
+    hicma_desc_destroy(W);
+    ctx->wave_min_tasks = rc ? saved : bi;
+    if (rc) return rc;
+    if (best) *best = bi;
+    if (best_seconds) *best_seconds = bs;
     return 0;
 }
 

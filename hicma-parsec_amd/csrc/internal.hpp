@@ -37,9 +37,8 @@ struct hicma_ctx {
     int *d_ovf;
     double *d_scalar;
     size_t lds_max;
-    // schedule: panel steps with fewer low-rank gemm tasks than this send
-    // the stacks of up to 60 columns to the 1024-thread recompression (one
-    // task per workgroup) instead of the one-wave kernels (0: never)
+    // schedule: panel steps with fewer low-rank gemm tasks than this run
+    // every task on the four-wave kernel (0: never)
     int wave_min_tasks;
     int profile;                     // record events around recompression launches
     std::vector<hipEvent_t> prof_ev; // pairs

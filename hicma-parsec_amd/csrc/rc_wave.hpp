@@ -379,7 +379,7 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
 // rb <= WV_SMAX: k_rc_wide); the order inside
 // a list is irrelevant (tasks of one step are independent).  Listed tasks are
 // marked -1 in tasks_s; every other task keeps its k_classify value.  With
-// wide_only, only list 3 is filled (the 1024-thread kernel takes the rest).
+// wide_only, every listed task goes to list 3 (four waves per task).
 __global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wcount, int cap, int wide_only)
 {
     const int T = t.NT - 1 - k;
@@ -393,8 +393,7 @@ __global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wco
     const int ra = *rank_at(t, m, k), rc = *rank_at(t, m, n), rb = *rank_at(t, n, k);
     const int s = rc + ra;
     if (ra > 0 && rb > 0 && s <= 128 && rb <= WV_SMAX) {
-        const int l = (s <= 32 && rb <= 32) ? 0 : ((s <= 48 && rb <= 48) ? 1 : (s <= WV_S2MAX ? 2 : 3));
-        if (wide_only && l < 3) return;
+        const int l = wide_only ? 3 : ((s <= 32 && rb <= 32) ? 0 : ((s <= 48 && rb <= 48) ? 1 : (s <= WV_S2MAX ? 2 : 3)));
         tasks_s[e] = -1;
         wlist[(size_t)l * cap + atomicAdd(wcount + l, 1)] = (int)e;
     }

@@ -34,7 +34,7 @@ EXPORTS = [
     "hicma_dense_logdet", "hicma_check_dpotrf2", "hicma_check_diff", "hicma_tlr_trsm", "hicma_tlr_potrs",
     "hicma_tlr_loglik", "hicma_dev_malloc", "hicma_dev_free", "hicma_memcpy", "hicma_tlr_loglik_noalloc",
     "hicma_init_ex", "hicma_server_create", "hicma_server_submit", "hicma_server_wait", "hicma_server_destroy",
-    "hicma_normal", "hicma_factor_graph", "hicma_set_wave_min_tasks",
+    "hicma_normal", "hicma_factor_graph", "hicma_set_wave_min_tasks", "hicma_autotune_schedule",
 ]
 
 _lib = None
@@ -124,6 +124,7 @@ def lib():
             "hicma_normal": (i, [u64, i, vp]),
             "hicma_factor_graph": (i, [vp, i]),
             "hicma_set_wave_min_tasks": (i, [vp, i]),
+            "hicma_autotune_schedule": (i, [vp, vp, d, i, i, C.POINTER(i), C.POINTER(d)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -463,6 +464,13 @@ class TLRMatrix:
                "hicma_tlr_loglik")
         X.close()
         return ld.value, qd.value, ll.value
+
+    def autotune_schedule(self, acc=1e-8, maxrank=None, reps=2):
+        """(threshold, seconds) chosen by hicma_autotune_schedule; kept in the context."""
+        b, sec = C.c_int(), C.c_double()
+        _check(lib().hicma_autotune_schedule(self.ctx.h, self.h, acc, maxrank or self.maxrank, reps, C.byref(b),
+                                             C.byref(sec)), "hicma_autotune_schedule")
+        return b.value, sec.value
 
     def use_graph(self, enable=True):
         """Capture the factorization in a HIP graph once, replay it after."""

@@ -139,13 +139,20 @@ int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr_desc *
 int hicma_factor_graph(hicma_tlr_desc *A, int enable);
 
 /* Schedule of the low-rank gemm + recompression tasks: panel steps with
- * fewer tasks than ntask give the stacks of up to 60 columns one 1024-thread
- * workgroup each instead of one wave (fewer tasks, shorter latency per task);
- * 0 (the default) keeps every step on the one-wave kernels.  No counterpart
+ * fewer tasks than ntask run every task on the four-wave kernel (a task's
+ * latency, not the step's throughput, bounds such steps); 0 (the default)
+ * sorts each step's tasks by stack width over the one-wave kernels.  No counterpart
  * in the reference (its PaRSEC runtime schedules tasks on CPU cores).
  * Results depend on the schedule only through rounding, and are the same for
  * the same schedule. */
 int hicma_set_wave_min_tasks(hicma_ctx *ctx, int ntask);
+
+/* Autotuner of that schedule: factors a copy of A (reps times per candidate:
+ * 0, 50, 100, 200, ... tasks) and keeps the fastest in ctx; A is unchanged.
+ * Returns the chosen threshold and its time.  Needs memory for one more
+ * descriptor of A's shape. */
+int hicma_autotune_schedule(hicma_ctx *ctx, hicma_tlr_desc *A, double acc, int maxrank, int reps, int *best,
+                            double *best_seconds);
 
 /* op_counts (hicma_parsec/op_counts.h:10-72), for callers that account work */
 unsigned long hicma_op_counts(char op, unsigned long a, unsigned long b, unsigned long c, unsigned long d);
