@@ -76,8 +76,10 @@ static void sketch(int ld, int ncol, uint64_t seed, double *om)
     }
 }
 #define SKETCH_SEED 0x5EED0FULL
-// width of the first generation sketch (ranks up to 54 are settled by it)
+// widths of the first two generation sketches (ranks up to 54 and 150 are
+// settled by them; wider ranks by the full maxrank + 10)
 #define GEN_FIRST_SKETCH 64
+#define GEN_SECOND_SKETCH 160
 
 // z ~ N(0, I): the same counter-based normal stream as the sketch
 extern "C" int hicma_normal(uint64_t seed, int n, double *z)
@@ -401,13 +403,17 @@ extern "C" int STARSH_gen(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr
         if ((rc = hicma_ensure_ws(ctx, g.stride * sizeof(double) * ns))) return rc;
         size_t lds = sizeof(double) * (BLK_GEMM_SMEM + BLK_RED_SMEM + 2 * QRB * QRB + 3 * (size_t)p + 16);
         HIPCHK(hipFuncSetAttribute((const void *)k_compress, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        // narrow sketch first; tiles whose rank it cannot vouch for again at full width
-        const int pw = p < GEN_FIRST_SKETCH ? p : GEN_FIRST_SKETCH;
-        hipLaunchKernelGGL(k_compress, dim3(ns), dim3(BS), lds, ctx->stream, t, (const double *)d_om, ctx->ws, g,
-                           pw, 0, tol, maxrank, ctx->d_info);
-        if (pw < p)
+        // narrow sketch first; tiles whose rank it cannot vouch for again at
+        // a wider one, up to the full width
+        const int ladder[3] = {GEN_FIRST_SKETCH, GEN_SECOND_SKETCH, p};
+        int prev = 0;
+        for (int pass = 0; pass < 3; pass++) {
+            const int pw = ladder[pass] < p ? ladder[pass] : p;
+            if (pw <= prev) continue;
             hipLaunchKernelGGL(k_compress, dim3(ns), dim3(BS), lds, ctx->stream, t, (const double *)d_om, ctx->ws,
-                               g, p, 1, tol, maxrank, ctx->d_info);
+                               g, pw, prev > 0 ? 1 : 0, tol, maxrank, ctx->d_info);
+            prev = pw;
+        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->stream));
         HIPCHK(ctx_free(ctx, d_om));
@@ -483,9 +489,11 @@ extern "C" int parsec_rank_check(hicma_ctx *ctx, hicma_tlr_desc *Ar, int band_si
 }
 
 // band_size_auto_tuning.jdf:201-318 (op-count comparison per sub-diagonal)
-extern "C" int hicma_band_autotune(const int *G, int NT, int NB)
+extern "C" int hicma_band_autotune(const int *G, int NT, int NB) { return hicma_band_autotune_weighted(G, NT, NB, 1.0); }
+
+extern "C" int hicma_band_autotune_weighted(const int *G, int NT, int NB, double lr_weight)
 {
-    if (!G || NT <= 0 || NB <= 0) return HICMA_ERR_ARG;
+    if (!G || NT <= 0 || NB <= 0 || !(lr_weight > 0)) return HICMA_ERR_ARG;
     const double fluct = 0.66666667;
     int opt_min = 0, ind_min = 0;
     for (int b = 2; b < NT; b++) {
@@ -493,7 +501,7 @@ extern "C" int hicma_band_autotune(const int *G, int NT, int NB)
         for (int m = b - 1; m < NT; m++) {
             int k = m + 1 - b;
             int ar = G[(size_t)k * NT + m];
-            tlr += (long long)NB * NB * ar;
+            tlr += (long long)(lr_weight * (double)NB * NB * ar);
             dense += (long long)NB * NB * NB;
         }
         for (int k = 0; k <= NT - 3; k++)
@@ -506,7 +514,7 @@ extern "C" int hicma_band_autotune(const int *G, int NT, int NB)
                                     hicma_op_counts('r', s, s, 2, 0) + hicma_op_counts('s', s, 0, 0, 0) +
                                     (unsigned long)s * cr + hicma_op_counts('o', NB, cr, s, 1) +
                                     hicma_op_counts('o', cr, NB, s, 2);
-                tlr += (long long)ops;
+                tlr += (long long)(lr_weight * (double)ops);
                 if (m - k < b) dense += 2LL * NB * NB * NB;
                 else if (n - k < b && m - n < b) dense += 4LL * NB * NB * ar;
                 else if (m - n < b) dense += 2LL * NB * NB * (ar < br ? ar : br) + 4LL * NB * ar * br;

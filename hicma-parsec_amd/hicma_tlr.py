@@ -25,7 +25,8 @@ ERR = {-1: "uplo must be Lower", -2: "maxrank > NB/2", -3: "maxrank > compmaxran
 EXPORTS = [
     "hicma_init", "hicma_fini", "hicma_desc_create", "hicma_desc_destroy", "hicma_desc_info",
     "hicma_get_points", "hicma_points", "STARSH_gen", "parsec_rank_gather", "parsec_rank_check",
-    "HICMA_get_stat2", "parsec_band_size_auto_tuning", "hicma_band_autotune", "parsec_band_regenerate",
+    "HICMA_get_stat2", "parsec_band_size_auto_tuning", "hicma_band_autotune", "hicma_band_autotune_weighted",
+    "parsec_band_regenerate",
     "HiCMA_dpotrf_L", "hicma_op_counts", "hicma_op_stats", "hicma_rank_overflows", "STARSH_check",
     "hicma_get_tile", "hicma_set_tile", "hicma_get_rank_map", "hicma_logdet", "hicma_recompress_stats",
     "hicma_profile", "hicma_profile_read", "hicma_desc_copy", "hicma_jacobi_unconverged", "hicma_exec_op_stats",
@@ -78,6 +79,7 @@ def lib():
             "HICMA_get_stat2": (None, [ip, i, i, C.POINTER(i), C.POINTER(i), C.POINTER(d)]),
             "parsec_band_size_auto_tuning": (i, [vp, vp, vp, ip, i]),
             "hicma_band_autotune": (i, [ip, i, i]),
+            "hicma_band_autotune_weighted": (i, [ip, i, i, d]),
             "parsec_band_regenerate": (i, [vp, i, vp, vp, vp, vp, i]),
             "HiCMA_dpotrf_L": (i, [vp, i, vp, vp, vp, d, i, i, C.POINTER(i), i, i, i, i, C.POINTER(i),
                                    vp, vp, vp, i]),
@@ -150,9 +152,17 @@ def op_counts(op, a, b=0, c=0, d=0):
     return lib().hicma_op_counts(op.encode() if isinstance(op, str) else op, a, b, c, d)
 
 
-def band_autotune(G, NT, NB):
+def band_autotune(G, NT, NB, lr_weight=None):
+    """hicma_band_autotune (the reference's rule), or with the low-rank op
+    counts weighted (hicma_band_autotune_weighted)."""
     G = np.ascontiguousarray(G, dtype=np.int32)
-    return lib().hicma_band_autotune(G.ctypes.data, NT, NB)
+    if lr_weight is None:
+        return lib().hicma_band_autotune(G.ctypes.data, NT, NB)
+    return lib().hicma_band_autotune_weighted(G.ctypes.data, NT, NB, float(lr_weight))
+
+
+# weight of the low-rank op counts in the serving loop's band choice
+SERVE_LR_WEIGHT = 8.0
 
 
 def rank_stat(G, NT, band):

@@ -110,6 +110,11 @@ int parsec_band_size_auto_tuning(hicma_ctx *ctx, hicma_tlr_desc *Ar, hicma_tlr_d
                                  int *rank_array, int NB);
 /* the same decision on a host rank map, without a descriptor */
 int hicma_band_autotune(const int *rank_array, int NT, int NB);
+/* The same choice with the low-rank op counts weighted by lr_weight (1 is
+ * the reference's rule): a cost model for a device where the dense band's
+ * gemms run on the matrix cores many times faster per op than the low-rank
+ * recompressions.  No counterpart in the reference. */
+int hicma_band_autotune_weighted(const int *rank_array, int NT, int NB, double lr_weight);
 /* parsec_band_regenerate (hicma_parsec/hicma_parsec.h:108-121): dense band
  * tiles regenerated in HBM; data / kernel / index as for STARSH_gen */
 int parsec_band_regenerate(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, void *data, hicma_kernel *kernel,

@@ -80,3 +80,27 @@ def test_driver_usage():
     assert r.returncode == 0 and "--fixedacc" in r.stderr
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
+
+
+def test_weighted_band_autotune():
+    """Weight 1 is the reference's rule; a heavier low-rank weight never
+    narrows the band; a weight so large that any low-rank work loses makes
+    every tile with nonzero rank dense (band NT)."""
+    rng = np.random.default_rng(11)
+    for NT, NB in [(10, 270), (30, 900), (64, 512)]:
+        for scale in (1, 5, 20, 60, 160):
+            # ranks decaying away from the diagonal, as covariance tiles do
+            G = np.zeros((NT, NT), np.int32)
+            for m in range(NT):
+                for n in range(m):
+                    G[n, m] = int(rng.integers(0, scale + 1) * np.exp(-(m - n) / 4.0))
+            Gm = G.ravel()
+            b1 = H.band_autotune(Gm, NT, NB)
+            assert H.band_autotune(Gm, NT, NB, 1.0) == b1
+            prev = b1
+            for w in (2.0, 8.0, 64.0):
+                bw = H.band_autotune(Gm, NT, NB, w)
+                assert bw >= prev or bw == NT, (NT, NB, scale, w, bw, prev)
+                prev = bw
+    full = np.full(100, 20, np.int32)
+    assert H.band_autotune(full, 10, 270, 1e9) == 10
