@@ -66,28 +66,34 @@ __device__ __forceinline__ wv_d4 mfma16(double a, double b, wv_d4 c)
 
 // Tall product D (NB x nc, ld NB) = alpha * A (NB x kd) K (kd x nc) with A's
 // column c at acol(c) and K in LDS (K[c][j] at Kl[c * ldk + j], zero for
-// j >= nc).  One 16-row tile at a time (tiles t0, t0 + tstep, ... when several
-// waves share the product), all its A loads issued up front;
-// the K operands are read from LDS.  Rows of a tile are read before any of
-// them is written, so D may overwrite A's storage row for row.
-template <int SM, typename ACol>
+// j >= nc).  RT 16-row tiles at a time (tiles t0, t0 + tstep, ... when several
+// waves share the product), all their A loads issued before the first
+// MFMA, so a wave keeps RT tiles of loads in flight; the K operands are read
+// from LDS.  Rows of a batch are read before any of them is written, so D
+// may overwrite A's storage row for row.
+template <int SM, int RT = (SM <= 32 ? 4 : (SM <= 64 ? 2 : 1)), typename ACol>
 __device__ __forceinline__ void wv_tall_mm(int NB, int kd, int nc, double alpha, ACol acol, const double *Kl,
                                            int ldk, double *D, int t0 = 0, int tstep = 1)
 {
     const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
     constexpr int KS = SM / 4, TN = SM / 16;
 #pragma unroll 1
-    for (int r0 = 16 * t0; r0 < NB; r0 += 16 * tstep) {  // row tiles t0, t0 + tstep, ...
-        const int row = r0 + i16;
-        double av[KS];
+    for (int r0 = 16 * t0; r0 < NB; r0 += 16 * tstep * RT) {  // batches of RT row tiles
+        double av[RT][KS];
 #pragma unroll
-        for (int kk = 0; kk < KS; kk++) {
-            const int c = 4 * kk + k4;
-            av[kk] = (4 * kk < kd && c < kd && row < NB) ? acol(c)[row] : 0.0;
+        for (int t = 0; t < RT; t++) {
+            const int row = r0 + 16 * tstep * t + i16;
+#pragma unroll
+            for (int kk = 0; kk < KS; kk++) {
+                const int c = 4 * kk + k4;
+                av[t][kk] = (4 * kk < kd && c < kd && row < NB) ? acol(c)[row] : 0.0;
+            }
         }
-        wv_d4 acc[TN];
+        wv_d4 acc[RT][TN];
 #pragma unroll
-        for (int tn = 0; tn < TN; tn++) acc[tn] = wv_d4{0.0, 0.0, 0.0, 0.0};
+        for (int t = 0; t < RT; t++)
+#pragma unroll
+            for (int tn = 0; tn < TN; tn++) acc[t][tn] = wv_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int kk = 0; kk < KS; kk++)
             if (4 * kk < kd) {
@@ -96,17 +102,22 @@ __device__ __forceinline__ void wv_tall_mm(int NB, int kd, int nc, double alpha,
                 for (int tn = 0; tn < TN; tn++)
                     if (16 * tn < nc) {
                         const double kb = (kr < kd) ? Kl[kr * ldk + 16 * tn + i16] : 0.0;
-                        acc[tn] = mfma16(av[kk], kb, acc[tn]);
+#pragma unroll
+                        for (int t = 0; t < RT; t++) acc[t][tn] = mfma16(av[t][kk], kb, acc[t][tn]);
                     }
             }
 #pragma unroll
-        for (int tn = 0; tn < TN; tn++) {
-            if (16 * tn < nc) {
-                const int j = 16 * tn + i16;
+        for (int t = 0; t < RT; t++) {
+            const int rt = r0 + 16 * tstep * t;
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int i = r0 + k4 + 4 * r;
-                    if (j < nc && i < NB) D[i + (size_t)j * NB] = alpha * acc[tn][r];
+            for (int tn = 0; tn < TN; tn++) {
+                if (16 * tn < nc) {
+                    const int j = 16 * tn + i16;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int i = rt + k4 + 4 * r;
+                        if (j < nc && i < NB) D[i + (size_t)j * NB] = alpha * acc[t][tn][r];
+                    }
                 }
             }
         }
