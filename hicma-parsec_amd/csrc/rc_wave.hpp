@@ -690,37 +690,43 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
                 continue;
             }
             wv_sync();
-            // ---- outputs.  V' = Q Kv first (Kv in LDS) into Out; then Ku is
-            //      staged into LDS and U' = P Ku is written straight over the
-            //      tile (a 16-row tile of P is read before its U' rows are
-            //      written); finally V' moves next to U'. ----
-            wv_tall_mm<SM>(NB, s, nr, 1.0,
-                           [&](int c) { return c < rc ? C + (size_t)(rc + c) * NB : Qn + (size_t)(c - rc) * NB; },
-                           Vl, SM, Out + (size_t)nr * NB);
-            wv_sync();
-            for (int q = lane; q < s * SM; q += 64) {
-                const int c = q / SM, j = q - c * SM;
-                Vl[q] = (j < nr) ? Ku[c * SM + j] : 0.0;
-            }
-            wv_sync();
-            wv_tall_mm<SM>(NB, s, nr, 1.0,
-                           [&](int c) { return c < rc ? (const double *)C + (size_t)c * NB : Au + (size_t)(c - rc) * NB; },
-                           Vl, SM, C);
-            wv_sync();
-            // V' from Out into columns nr .. 2nr-1 of the tile (flat copy)
-            {
-                const double *src = Out + (size_t)nr * NB;
-                double *d2 = C + (size_t)nr * NB;
-                const int tot = nr * NB;
-                int q = lane;
-                for (; q + 64 * 7 < tot; q += 64 * 8) {
-                    double v[8];
-#pragma unroll
-                    for (int u = 0; u < 8; u++) v[u] = src[q + 64 * u];
-#pragma unroll
-                    for (int u = 0; u < 8; u++) d2[q + 64 * u] = v[u];
+            // ---- outputs, written in place into the tile.  A product reads a
+            //      batch of rows of all its sources before writing those rows,
+            //      and every column of the tile has stride NB, so an output
+            //      may overwrite its own sources.  The two products are
+            //      ordered so that neither overwrites the other's sources:
+            //      U' = P Ku (columns 0 .. nr-1, sources Cu = columns
+            //      0 .. rc-1) and V' = Q Kv (columns nr .. 2nr-1, sources
+            //      Cv = columns rc .. 2rc-1): U' first when nr <= rc, V'
+            //      first otherwise. ----
+            auto v_out = [&]() {  // Kv in LDS
+                wv_tall_mm<SM>(NB, s, nr, 1.0,
+                               [&](int c) {
+                                   return c < rc ? (const double *)C + (size_t)(rc + c) * NB : Qn + (size_t)(c - rc) * NB;
+                               },
+                               Vl, SM, C + (size_t)nr * NB);
+                wv_sync();
+            };
+            auto u_out = [&]() {  // stages Ku into LDS
+                for (int q = lane; q < s * SM; q += 64) {
+                    const int c = q / SM, j = q - c * SM;
+                    Vl[q] = (j < nr) ? Ku[c * SM + j] : 0.0;
                 }
-                for (; q < tot; q += 64) d2[q] = src[q];
+                wv_sync();
+                wv_tall_mm<SM>(NB, s, nr, 1.0,
+                               [&](int c) { return c < rc ? (const double *)C + (size_t)c * NB : Au + (size_t)(c - rc) * NB; },
+                               Vl, SM, C);
+                wv_sync();
+            };
+            if (nr > rc) {
+                v_out();
+                u_out();
+            } else {
+                for (int q = lane; q < s * SM; q += 64) Out[q] = Vl[q];  // Kv aside while Ku uses the LDS
+                u_out();
+                for (int q = lane; q < s * SM; q += 64) Vl[q] = Out[q];
+                wv_sync();
+                v_out();
             }
         }
         if (lane == 0) {
@@ -1215,35 +1221,38 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
                 __syncthreads();
                 continue;
             }
-            // ---- outputs as in k_rc_wave: V' to Out + nr NB, then U' in place
-            //      over C (each wave reads and writes only its own row tiles) ----
-            wv_tall_mm<WB_SM>(NB, s, nr, 1.0,
-                              [&](int c) { return c < rc ? C + (size_t)(rc + c) * NB : Qn + (size_t)(c - rc) * NB; },
-                              Vl, WB_SM, Out + (size_t)nr * NB, wid, WB_WAVES);
-            __syncthreads();
-            for (int q = tid; q < s * WB_SM; q += NTH) {
-                const int c = q / WB_SM, j = q - c * WB_SM;
-                Vl[q] = (j < nr) ? Ku[c * WB_SM + j] : 0.0;
-            }
-            __syncthreads();
-            wv_tall_mm<WB_SM>(
-                NB, s, nr, 1.0,
-                [&](int c) { return c < rc ? (const double *)C + (size_t)c * NB : Au + (size_t)(c - rc) * NB; }, Vl,
-                WB_SM, C, wid, WB_WAVES);
-            __syncthreads();
-            {
-                const double *src = Out + (size_t)nr * NB;
-                double *d2 = C + (size_t)nr * NB;
-                const int tot = nr * NB;
-                int q = tid;
-                for (; q + NTH * 3 < tot; q += NTH * 4) {
-                    double v[4];
-#pragma unroll
-                    for (int u = 0; u < 4; u++) v[u] = src[q + NTH * u];
-#pragma unroll
-                    for (int u = 0; u < 4; u++) d2[q + NTH * u] = v[u];
+            // ---- outputs in place into the tile, ordered as in k_rc_wave (each
+            //      wave reads and writes only its own row tiles) ----
+            auto v_out = [&]() {  // Kv in LDS
+                wv_tall_mm<WB_SM>(NB, s, nr, 1.0,
+                                  [&](int c) {
+                                      return c < rc ? (const double *)C + (size_t)(rc + c) * NB
+                                                    : Qn + (size_t)(c - rc) * NB;
+                                  },
+                                  Vl, WB_SM, C + (size_t)nr * NB, wid, WB_WAVES);
+                __syncthreads();
+            };
+            auto u_out = [&]() {  // stages Ku into LDS
+                for (int q = tid; q < s * WB_SM; q += NTH) {
+                    const int c = q / WB_SM, j = q - c * WB_SM;
+                    Vl[q] = (j < nr) ? Ku[c * WB_SM + j] : 0.0;
                 }
-                for (; q < tot; q += NTH) d2[q] = src[q];
+                __syncthreads();
+                wv_tall_mm<WB_SM>(
+                    NB, s, nr, 1.0,
+                    [&](int c) { return c < rc ? (const double *)C + (size_t)c * NB : Au + (size_t)(c - rc) * NB; },
+                    Vl, WB_SM, C, wid, WB_WAVES);
+                __syncthreads();
+            };
+            if (nr > rc) {
+                v_out();
+                u_out();
+            } else {
+                for (int q = tid; q < s * WB_SM; q += NTH) Out[q] = Vl[q];  // Kv aside while Ku uses the LDS
+                u_out();
+                for (int q = tid; q < s * WB_SM; q += NTH) Vl[q] = Out[q];
+                __syncthreads();
+                v_out();
             }
         }
         if (tid == 0) {
