@@ -644,7 +644,7 @@ static int launch_potrf(hicma_tlr_desc *A, int k, double *Winv, double *Linv, in
     hipLaunchKernelGGL(k_fill_i32, dim3((nb * nb + 2 + 255) / 256), dim3(256), 0, s, dflags, nb * nb + 2, 0);
     // one workgroup per block column: a whole column of tasks can run at once
     hipLaunchKernelGGL(k_potrf_dag, dim3(nb), dim3(BS), sizeof(double) * DAG_SMEM, s, t, k, Winv, Linv, dflags,
-                       A->ctx->d_info, o);
+                       A->ctx->d_info, A->ctx->d_info + 3, o);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -952,6 +952,10 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     for (int i = 0; i < 12; i++) A->last_stats[i] = (unsigned long)stats[i];
     A->last_ovf = hinfo[1];
     A->last_unconv = hinfo[2];
+    if (hinfo[3]) {
+        fprintf(stderr, "hicma_tlr: a wait of the diagonal-tile schedule ran out; the factor is not valid\n");
+        return HICMA_ERR_HIP;
+    }
     if (tileopcounters) {
         std::vector<unsigned long long> to((size_t)NT * NT);
         HIPCHK(hipMemcpy(to.data(), A->d_tileops, sizeof(unsigned long long) * to.size(), hipMemcpyDeviceToHost));

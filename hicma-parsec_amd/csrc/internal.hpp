@@ -33,7 +33,8 @@ struct hicma_ctx {
     hipStream_t stream;
     double *ws;
     size_t ws_bytes;
-    int *d_info;   // [0] info, [1] rank overflows, [2] unconverged Jacobi SVDs
+    int *d_info;   // [0] info, [1] rank overflows, [2] unconverged Jacobi SVDs,
+                   // [3] a bounded device wait ran out
     int *d_ovf;
     double *d_scalar;
     size_t lds_max;

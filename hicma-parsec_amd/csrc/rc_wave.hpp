@@ -752,16 +752,21 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
 #define WB_SM 128                       // largest stack width of the wide variant
 #define WB_LD (WB_SM + 1)               // LDS column stride of the Jacobi matrix
 #define WB_LDS (WB_SM * WB_LD + 2 * WB_SM)  // doubles of LDS (one wave per workgroup)
+#define WB_QR_SH (64 * (WV_KMAX * WV_PW + 1))  // doubles of LDS per QR side (two sides fit in WB_LDS)
 
 __host__ __device__ inline size_t wb_slot_doubles(int NB)
 {
     return (size_t)NB * 4 * WB_SM + 3 * (size_t)WB_SM * WB_SM;
 }
 
-// wv_geqrf_r for s <= 128: pivot row j lives on lane j & 63 of row block
-// j >> 6 (0 or 1); rows of block 0 are above every pivot of block 1.
+// wv_geqrf_r for s <= 128 on two waves: pivot row j lives on lane j & 63 of
+// row block j >> 6 (0 or 1); rows of block 0 are above every pivot of block 1.
+// The lead wave (part 0) factors each panel and passes its reflectors and T
+// to the helper (part 1) through `sh` (64 (16 WV_PW + 1) doubles of LDS);
+// the two then update alternate trailing columns.  Every wave of the
+// workgroup must call it with the same s (the barriers are workgroup-wide).
 // R is written column-major (R(i, j) at Rout[i + j * WB_SM]).
-__device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout)
+__device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout, int part, double *sh)
 {
     const int lane = threadIdx.x & 63;
     const int NB = A.NB;
@@ -770,105 +775,139 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout)
         const bool first = (j0 == 0);
         const int ps = j0 >> 6;  // row block of the panel's pivots (panels never straddle 64)
         double a[WV_KMAX][WV_PW];
-#pragma unroll
-        for (int c = 0; c < WV_PW; c++) {
-            const double *col = A.col(j0 + c, first);
-#pragma unroll
-            for (int k = 0; k < WV_KMAX; k++) {
-                const int i = lane + 64 * k;
-                a[k][c] = (c < jb && i < NB) ? col[i] : 0.0;
-            }
-        }
         double T[WV_PW][WV_PW];
-#pragma unroll
-        for (int r = 0; r < WV_PW; r++)
-#pragma unroll
-            for (int c = 0; c < WV_PW; c++) T[r][c] = 0.0;
-#pragma unroll
-        for (int jj = 0; jj < WV_PW; jj++) {
-            if (jj < jb) {
-                const int jl = (j0 + jj) & 63;
-                const double x0 = a[0][jj], x1 = a[1][jj];
-                // part of column jj strictly below the pivot, per row block
-                const double y0 = (ps == 0 && lane > jl) ? x0 : 0.0;
-                const double y1 = (ps == 0 || lane > jl) ? x1 : 0.0;
-                double p[WV_PW];
-#pragma unroll
-                for (int c = 0; c < WV_PW; c++)
-                    p[c] = y0 * (c == jj ? y0 : a[0][c]) + y1 * (c == jj ? y1 : a[1][c]);
-#pragma unroll
-                for (int k = 2; k < WV_KMAX; k++)
-#pragma unroll
-                    for (int c = 0; c < WV_PW; c++) p[c] = fma(a[k][jj], a[k][c], p[c]);
-                const double red = wrsN<WV_PW>(p);
-                double S[WV_PW], prow[WV_PW];
-#pragma unroll
-                for (int c = 0; c < WV_PW; c++) {
-                    S[c] = rdlane(red, (64 / WV_PW) * c);
-                    prow[c] = rdlane(ps ? a[1][c] : a[0][c], jl);
-                }
-                const double alpha = prow[jj], xn2 = S[jj];
-                double tau = 0.0, beta = alpha, scal = 0.0;
-                if (xn2 > 0.0) {
-                    beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-                    tau = (beta - alpha) / beta;
-                    scal = 1.0 / (alpha - beta);
-                }
-                const double v0 = (ps == 0 && lane == jl) ? 1.0 : y0 * scal;
-                const double v1 = (ps == 1 && lane == jl) ? 1.0 : y1 * scal;
-                a[0][jj] = v0;
-                a[1][jj] = v1;
-#pragma unroll
-                for (int k = 2; k < WV_KMAX; k++) a[k][jj] *= scal;
-#pragma unroll
-                for (int c = 0; c < WV_PW; c++) {
-                    if (c > jj) {
-                        const double td = tau * (prow[c] + S[c] * scal);
-#pragma unroll
-                        for (int k = 0; k < WV_KMAX; k++) a[k][c] = fma(-td, a[k][jj], a[k][c]);
-                    }
-                }
-                // R entries above the pivot and beta at the pivot; reflector below
-                if (ps == 0) {
-                    a[0][jj] = (lane == jl) ? beta : ((lane > jl) ? v0 : x0);
-                } else {
-                    a[0][jj] = x0;
-                    a[1][jj] = (lane == jl) ? beta : ((lane > jl) ? v1 : x1);
-                }
-#pragma unroll
-                for (int l = 0; l < WV_PW; l++) {
-                    if (l < jj) {
-                        double acc = 0.0;
-#pragma unroll
-                        for (int q = 0; q < WV_PW; q++)
-                            if (q >= l && q < jj) acc = fma(T[l][q], prow[q] + S[q] * scal, acc);
-                        T[l][jj] = -tau * acc;
-                    }
-                }
-                T[jj][jj] = tau;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < WV_PW; c++)
-            if (c < jb) {
-#pragma unroll
-                for (int blk = 0; blk < 2; blk++) {
-                    const int i = lane + 64 * blk;
-                    if (i < s) Rout[i + (size_t)(j0 + c) * WB_SM] = (i <= j0 + c) ? a[blk][c] : 0.0;
-                }
-            }
-        // reflector entries of row blocks 0 and 1 (1 at the pivot, 0 above)
         double m0[WV_PW], m1[WV_PW];
+        if (part == 0) {
 #pragma unroll
-        for (int c = 0; c < WV_PW; c++) {
-            const int gl = (j0 + c) & 63;
-            const bool in = c < jb;
-            m0[c] = (!in || ps == 1) ? 0.0 : ((lane == gl) ? 1.0 : (lane > gl ? a[0][c] : 0.0));
-            m1[c] = !in ? 0.0 : (ps == 0 ? a[1][c] : ((lane == gl) ? 1.0 : (lane > gl ? a[1][c] : 0.0)));
+            for (int c = 0; c < WV_PW; c++) {
+                const double *col = A.col(j0 + c, first);
+#pragma unroll
+                for (int k = 0; k < WV_KMAX; k++) {
+                    const int i = lane + 64 * k;
+                    a[k][c] = (c < jb && i < NB) ? col[i] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < WV_PW; r++)
+#pragma unroll
+                for (int c = 0; c < WV_PW; c++) T[r][c] = 0.0;
+#pragma unroll
+            for (int jj = 0; jj < WV_PW; jj++) {
+                if (jj < jb) {
+                    const int jl = (j0 + jj) & 63;
+                    const double x0 = a[0][jj], x1 = a[1][jj];
+                    // part of column jj strictly below the pivot, per row block
+                    const double y0 = (ps == 0 && lane > jl) ? x0 : 0.0;
+                    const double y1 = (ps == 0 || lane > jl) ? x1 : 0.0;
+                    double p[WV_PW];
+#pragma unroll
+                    for (int c = 0; c < WV_PW; c++)
+                        p[c] = y0 * (c == jj ? y0 : a[0][c]) + y1 * (c == jj ? y1 : a[1][c]);
+#pragma unroll
+                    for (int k = 2; k < WV_KMAX; k++)
+#pragma unroll
+                        for (int c = 0; c < WV_PW; c++) p[c] = fma(a[k][jj], a[k][c], p[c]);
+                    const double red = wrsN<WV_PW>(p);
+                    double S[WV_PW], prow[WV_PW];
+#pragma unroll
+                    for (int c = 0; c < WV_PW; c++) {
+                        S[c] = rdlane(red, (64 / WV_PW) * c);
+                        prow[c] = rdlane(ps ? a[1][c] : a[0][c], jl);
+                    }
+                    const double alpha = prow[jj], xn2 = S[jj];
+                    double tau = 0.0, beta = alpha, scal = 0.0;
+                    if (xn2 > 0.0) {
+                        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+                        tau = (beta - alpha) / beta;
+                        scal = 1.0 / (alpha - beta);
+                    }
+                    const double v0 = (ps == 0 && lane == jl) ? 1.0 : y0 * scal;
+                    const double v1 = (ps == 1 && lane == jl) ? 1.0 : y1 * scal;
+                    a[0][jj] = v0;
+                    a[1][jj] = v1;
+#pragma unroll
+                    for (int k = 2; k < WV_KMAX; k++) a[k][jj] *= scal;
+#pragma unroll
+                    for (int c = 0; c < WV_PW; c++) {
+                        if (c > jj) {
+                            const double td = tau * (prow[c] + S[c] * scal);
+#pragma unroll
+                            for (int k = 0; k < WV_KMAX; k++) a[k][c] = fma(-td, a[k][jj], a[k][c]);
+                        }
+                    }
+                    // R entries above the pivot and beta at the pivot; reflector below
+                    if (ps == 0) {
+                        a[0][jj] = (lane == jl) ? beta : ((lane > jl) ? v0 : x0);
+                    } else {
+                        a[0][jj] = x0;
+                        a[1][jj] = (lane == jl) ? beta : ((lane > jl) ? v1 : x1);
+                    }
+#pragma unroll
+                    for (int l = 0; l < WV_PW; l++) {
+                        if (l < jj) {
+                            double acc = 0.0;
+#pragma unroll
+                            for (int q = 0; q < WV_PW; q++)
+                                if (q >= l && q < jj) acc = fma(T[l][q], prow[q] + S[q] * scal, acc);
+                            T[l][jj] = -tau * acc;
+                        }
+                    }
+                    T[jj][jj] = tau;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++)
+                if (c < jb) {
+#pragma unroll
+                    for (int blk = 0; blk < 2; blk++) {
+                        const int i = lane + 64 * blk;
+                        if (i < s) Rout[i + (size_t)(j0 + c) * WB_SM] = (i <= j0 + c) ? a[blk][c] : 0.0;
+                    }
+                }
+            // reflector entries of row blocks 0 and 1 (1 at the pivot, 0 above)
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++) {
+                const int gl = (j0 + c) & 63;
+                const bool in = c < jb;
+                m0[c] = (!in || ps == 1) ? 0.0 : ((lane == gl) ? 1.0 : (lane > gl ? a[0][c] : 0.0));
+                m1[c] = !in ? 0.0 : (ps == 0 ? a[1][c] : ((lane == gl) ? 1.0 : (lane > gl ? a[1][c] : 0.0)));
+            }
+
+            // the panel's reflectors and T for the helper wave (value-major:
+            // consecutive lanes, consecutive banks)
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++) {
+                sh[(0 * WV_PW + c) * 64 + lane] = m0[c];
+                sh[(1 * WV_PW + c) * 64 + lane] = m1[c];
+#pragma unroll
+                for (int k = 2; k < WV_KMAX; k++) sh[(k * WV_PW + c) * 64 + lane] = a[k][c];
+            }
+            if (lane == 0) {  // T is the same on every lane
+#pragma unroll
+                for (int r = 0; r < WV_PW; r++)
+#pragma unroll
+                    for (int c = 0; c < WV_PW; c++) sh[WV_KMAX * WV_PW * 64 + r * WV_PW + c] = T[r][c];
+            }
         }
+        __syncthreads();
+        if (part == 1) {
+#pragma unroll
+            for (int c = 0; c < WV_PW; c++) {
+                m0[c] = sh[(0 * WV_PW + c) * 64 + lane];
+                m1[c] = sh[(1 * WV_PW + c) * 64 + lane];
+#pragma unroll
+                for (int k = 2; k < WV_KMAX; k++) a[k][c] = sh[(k * WV_PW + c) * 64 + lane];
+            }
+#pragma unroll
+            for (int r = 0; r < WV_PW; r++)
+#pragma unroll
+                for (int c = 0; c < WV_PW; c++) T[r][c] = sh[WV_KMAX * WV_PW * 64 + r * WV_PW + c];
+        }
+        // trailing columns j0 + jb + part, + 2, ...: the two waves of the
+        // side take alternate columns
         double nb_[WV_KMAX];
         {
-            const int t = j0 + jb;
+            const int t = j0 + jb + part;
             const double *col = A.col(t < s ? t : j0, first);
 #pragma unroll
             for (int k = 0; k < WV_KMAX; k++) {
@@ -876,13 +915,13 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout)
                 nb_[k] = (t < s && i < NB) ? col[i] : 0.0;
             }
         }
-        for (int t = j0 + jb; t < s; t++) {
+        for (int t = j0 + jb + part; t < s; t += 2) {
             double b[WV_KMAX];
 #pragma unroll
             for (int k = 0; k < WV_KMAX; k++) b[k] = nb_[k];
             {
-                const bool nx = t + 1 < s;
-                const double *col = A.col(nx ? t + 1 : t, first);
+                const bool nx = t + 2 < s;
+                const double *col = A.col(nx ? t + 2 : t, first);
 #pragma unroll
                 for (int k = 0; k < WV_KMAX; k++) {
                     const int i = lane + 64 * k;
@@ -927,6 +966,7 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout)
                 if (i < NB) out[i] = b[k];
             }
         }
+        __syncthreads();  // both waves' trailing columns are in W before the next panel
     }
 }
 
@@ -1017,11 +1057,14 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
         wv_tall_mm<WB_SM>(NB, rb, ra, -1.0, [&](int c) { return Bu + (size_t)c * NB; }, L, WB_SM, Qn, wid,
                           WB_WAVES);
         __syncthreads();
-        // ---- R factors: P = [Cu Au] on wave 0 (trailing copy in Wq), Q = [Cv Qn]
-        //      on wave 1 (trailing copy in Out, unused until the outputs) ----
-        if (wid < 2) {
-            WvCols S{wid ? C + (size_t)NB * rc : C, wid ? (const double *)Qn : Au, rc, wid ? Out : Wq, NB};
-            wb_geqrf_r(s, S, wid ? Rq : Rp);
+        // ---- R factors: P = [Cu Au] on waves 0 and 2 (trailing copy in Wq),
+        //      Q = [Cv Qn] on waves 1 and 3 (trailing copy in Out, unused until
+        //      the outputs); the panels' reflectors pass through the LDS,
+        //      free until the Jacobi ----
+        {
+            const int side = wid & 1;
+            WvCols S{side ? C + (size_t)NB * rc : C, side ? (const double *)Qn : Au, rc, side ? Out : Wq, NB};
+            wb_geqrf_r(s, S, side ? Rq : Rp, wid >> 1, L + (size_t)side * WB_QR_SH);
         }
         __syncthreads();
         // ---- M^T = Rq Rp^T on the matrix cores (K = Rp^T = Rp's column-major

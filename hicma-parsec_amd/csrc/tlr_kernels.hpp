@@ -227,9 +227,11 @@ __device__ int diag_factor_wave(double *Lr, double *Wc, int jb)
 // atomic counter.  Every task waits only for tasks with smaller numbers,
 // which started earlier and wait for nothing larger, so the schedule cannot
 // deadlock; each wait is bounded all the same (a wait that runs out marks
-// flags[1] and the task goes on).
+// *err and the task goes on, so the kernel always ends).
 //
-// flags: [0] task counter, [1] wait ran out, [2 + task] done; zero at launch.
+// flags: [0] task counter, [1] unused, [2 + task] done; zero at launch.  A
+// wait that runs out sets *err (sticky over the factorization; the host
+// reports it as an error).
 // Publishing a block: its stores, an agent-scope release fence in every
 // thread (stores complete, L2 written back: another XCD may read the block),
 // a barrier and the done flag; reading: the flag, an agent-scope acquire
@@ -240,7 +242,7 @@ __device__ int diag_factor_wave(double *Lr, double *Wc, int jb)
 __device__ __forceinline__ int dag_fac(int i, int j, int nb) { return j * nb + (i - j); }
 __device__ __forceinline__ int dag_inv(int i, int j, int nb) { return i * nb + (nb - i) + j; }
 
-__device__ __forceinline__ void dag_wait2(int *flags, int t0, int t1)
+__device__ __forceinline__ void dag_wait2(int *flags, int t0, int t1, int *err)
 {
     if (threadIdx.x == 0) {
         int n = 0;
@@ -249,7 +251,7 @@ __device__ __forceinline__ void dag_wait2(int *flags, int t0, int t1)
                !__hip_atomic_load(f1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
             __builtin_amdgcn_s_sleep(2);
             if (++n > DAG_SPIN_LIMIT) {
-                __hip_atomic_store(flags + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
         }
@@ -270,7 +272,7 @@ __device__ __forceinline__ void dag_done(int *flags, int task)
 // Linv: nb x nb blocks of 64 x 64 (column major, ld 64) at Linv + (i nb + j) 4096,
 // written for i > j; the diagonal blocks are Winv's
 __global__ void __launch_bounds__(BS) k_potrf_dag(TLR t, int k, double *Winv, double *Linv, int *flags, int *info,
-                                                 OpLog o)
+                                                 int *err, OpLog o)
 {
     extern __shared__ double sm[];
     double *Lr = sm, *Wc = sm + PBK * PLR, *Mk = sm + 2 * PBK * PLR;
@@ -294,13 +296,13 @@ __global__ void __launch_bounds__(BS) k_potrf_dag(TLR t, int k, double *Winv, do
         if (inverse) {
             // S = sum_{l=j}^{i-1} L_il Linv_lj, then Linv_ij = -W_i S
             for (int l = j; l < i; l++) {
-                dag_wait2(flags, dag_fac(i, l, nb), l == j ? dag_fac(j, j, nb) : dag_inv(l, j, nb));
+                dag_wait2(flags, dag_fac(i, l, nb), l == j ? dag_fac(j, j, nb) : dag_inv(l, j, nb), err);
                 const double *blk = (l == j) ? Winv + (size_t)j * PBK * PBK : Linv + ((size_t)l * nb + j) * PBK * PBK;
                 const MkOps<false, true> op{A + r0, NB, blk - (long)PBK * l, PBK};
                 mk_accum(op, ri, cj, 0, 0, (long)PBK * l, (long)PBK * l + PBK, acc, Mk);
             }
             mk_store(acc, [&](int ii, int jj, double v) { Lr[ii * PLR + jj] = v; });  // S(ii, jj)
-            dag_wait2(flags, dag_fac(i, i, nb), dag_fac(i, i, nb));
+            dag_wait2(flags, dag_fac(i, i, nb), dag_fac(i, i, nb), err);
             const double *Wi = Winv + (size_t)i * PBK * PBK;
             for (int e = tid; e < PBK * PBK; e += BS) {
                 const int ii = e % PBK, l = e / PBK;  // W_i(ii, l), column major
@@ -321,7 +323,7 @@ __global__ void __launch_bounds__(BS) k_potrf_dag(TLR t, int k, double *Winv, do
             // S = A_ij - sum_l L_il L_jl^T
             const MkOps<false, false> op{A + r0, NB, A + c0, NB};
             for (int l = 0; l < j; l++) {
-                dag_wait2(flags, dag_fac(i, l, nb), dag_fac(j, l, nb));
+                dag_wait2(flags, dag_fac(i, l, nb), dag_fac(j, l, nb), err);
                 mk_accum(op, ri, cj, 0, 0, (long)PBK * l, (long)PBK * l + PBK, acc, Mk);
             }
             if (i == j) {
@@ -349,7 +351,7 @@ __global__ void __launch_bounds__(BS) k_potrf_dag(TLR t, int k, double *Winv, do
                 mk_store(acc, [&](int ii, int jj, double v) {
                     Lr[ii * PLR + jj] = (ii < ri && jj < cj) ? A[r0 + ii + (size_t)(c0 + jj) * NB] - v : 0.0;
                 });
-                dag_wait2(flags, dag_fac(j, j, nb), dag_fac(j, j, nb));
+                dag_wait2(flags, dag_fac(j, j, nb), dag_fac(j, j, nb), err);
                 const double *Wj = Winv + (size_t)j * PBK * PBK;
                 for (int e = tid; e < PBK * PBK; e += BS) {
                     const int ii = e % PBK, c = e / PBK;  // W(ii, c), column major
