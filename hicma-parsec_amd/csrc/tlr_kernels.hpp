@@ -234,30 +234,46 @@ __device__ int diag_factor_wave(double *Lr, double *Wc, int jb)
 // reports it as an error).
 // Publishing a block: its stores, an agent-scope release fence in every
 // thread (stores complete, L2 written back: another XCD may read the block),
-// a barrier and the done flag; reading: the flag, an agent-scope acquire
-// fence (L1 and L2 invalidated), a barrier.
+// a barrier and the done flag; reading: the flags, one agent-scope acquire
+// fence (L1 and L2 invalidated) for all dependencies already done, a barrier.
 #define DAG_SPIN_LIMIT (1 << 21)
 #define DAG_SMEM (2 * PBK * PLR + MK_SMEM)
 
 __device__ __forceinline__ int dag_fac(int i, int j, int nb) { return j * nb + (i - j); }
 __device__ __forceinline__ int dag_inv(int i, int j, int nb) { return i * nb + (nb - i) + j; }
 
-__device__ __forceinline__ void dag_wait2(int *flags, int t0, int t1, int *err)
+// Waits until tasks t0(l) and t1(l) are done, then counts how many of the
+// following chunks l+1 .. lend-1 are done as well, and acquires once for all
+// of them.  Returns the first chunk not yet acquired.  dep(l, &a, &b) names
+// chunk l's two tasks.
+template <typename Dep>
+__device__ __forceinline__ int dag_acquire(int *flags, int l, int lend, Dep dep, int *err)
 {
+    __shared__ int s_ready;
     if (threadIdx.x == 0) {
+        auto done = [&](int c) {
+            int a, b;
+            dep(c, a, b);
+            return __hip_atomic_load(flags + 2 + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &&
+                   __hip_atomic_load(flags + 2 + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        };
         int n = 0;
-        const int *f0 = flags + 2 + t0, *f1 = flags + 2 + t1;
-        while (!__hip_atomic_load(f0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-               !__hip_atomic_load(f1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        while (!done(l)) {
             __builtin_amdgcn_s_sleep(2);
             if (++n > DAG_SPIN_LIMIT) {
                 __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
         }
+        int r = l + 1;
+        while (r < lend && done(r)) r++;
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_ready = r;
     }
     __syncthreads();
+    const int r = s_ready;
+    __syncthreads();  // s_ready is reused by the next call
+    return r;
 }
 
 __device__ __forceinline__ void dag_done(int *flags, int task)
@@ -295,14 +311,19 @@ __global__ void __launch_bounds__(BS) k_potrf_dag(TLR t, int k, double *Winv, do
         mk_zero(acc);
         if (inverse) {
             // S = sum_{l=j}^{i-1} L_il Linv_lj, then Linv_ij = -W_i S
+            auto dep = [&](int c, int &a, int &b) {
+                a = dag_fac(i, c, nb);
+                b = (c == j) ? dag_fac(j, j, nb) : dag_inv(c, j, nb);
+            };
+            int ready = j;
             for (int l = j; l < i; l++) {
-                dag_wait2(flags, dag_fac(i, l, nb), l == j ? dag_fac(j, j, nb) : dag_inv(l, j, nb), err);
+                if (l >= ready) ready = dag_acquire(flags, l, i, dep, err);
                 const double *blk = (l == j) ? Winv + (size_t)j * PBK * PBK : Linv + ((size_t)l * nb + j) * PBK * PBK;
                 const MkOps<false, true> op{A + r0, NB, blk - (long)PBK * l, PBK};
                 mk_accum(op, ri, cj, 0, 0, (long)PBK * l, (long)PBK * l + PBK, acc, Mk);
             }
             mk_store(acc, [&](int ii, int jj, double v) { Lr[ii * PLR + jj] = v; });  // S(ii, jj)
-            dag_wait2(flags, dag_fac(i, i, nb), dag_fac(i, i, nb), err);
+            dag_acquire(flags, 0, 1, [&](int, int &a, int &b) { a = b = dag_fac(i, i, nb); }, err);
             const double *Wi = Winv + (size_t)i * PBK * PBK;
             for (int e = tid; e < PBK * PBK; e += BS) {
                 const int ii = e % PBK, l = e / PBK;  // W_i(ii, l), column major
@@ -322,8 +343,13 @@ __global__ void __launch_bounds__(BS) k_potrf_dag(TLR t, int k, double *Winv, do
         } else {
             // S = A_ij - sum_l L_il L_jl^T
             const MkOps<false, false> op{A + r0, NB, A + c0, NB};
+            auto dep = [&](int c, int &a, int &b) {
+                a = dag_fac(i, c, nb);
+                b = dag_fac(j, c, nb);
+            };
+            int ready = 0;
             for (int l = 0; l < j; l++) {
-                dag_wait2(flags, dag_fac(i, l, nb), dag_fac(j, l, nb), err);
+                if (l >= ready) ready = dag_acquire(flags, l, j, dep, err);
                 mk_accum(op, ri, cj, 0, 0, (long)PBK * l, (long)PBK * l + PBK, acc, Mk);
             }
             if (i == j) {
@@ -351,7 +377,7 @@ __global__ void __launch_bounds__(BS) k_potrf_dag(TLR t, int k, double *Winv, do
                 mk_store(acc, [&](int ii, int jj, double v) {
                     Lr[ii * PLR + jj] = (ii < ri && jj < cj) ? A[r0 + ii + (size_t)(c0 + jj) * NB] - v : 0.0;
                 });
-                dag_wait2(flags, dag_fac(j, j, nb), dag_fac(j, j, nb), err);
+                dag_acquire(flags, 0, 1, [&](int, int &a, int &b) { a = b = dag_fac(j, j, nb); }, err);
                 const double *Wj = Winv + (size_t)j * PBK * PBK;
                 for (int e = tid; e < PBK * PBK; e += BS) {
                     const int ii = e % PBK, c = e / PBK;  // W(ii, c), column major
