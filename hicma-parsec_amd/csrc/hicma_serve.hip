@@ -31,10 +31,11 @@
 
 size_t hicma_solve_bytes(int N, int NB);  // hicma_solve.hip
 
-// band choice of a request: low-rank op counts weighted against the dense
-// band's (the recompression kernels run at about 3 TFLOP/s, the band's
-// gemms on the matrix cores about eight times faster)
-#define SERVE_LR_WEIGHT 8.0
+// band choice of a request: the time model with this device's rates (dense
+// tiles about 10 TFLOP/s as the batched band kernels run, recompressions
+// about 3.5 TFLOP/s of reference op counts)
+#define SERVE_DENSE_TFLOPS 10.0
+#define SERVE_LR_TFLOPS 3.5
 
 namespace {
 
@@ -149,7 +150,7 @@ static hicma_lik_result run(hicma_server *S, hicma_ctx *ctx, SvRequest *q)
         const int NT = (r.N + r.NB - 1) / r.NB;
         std::vector<int> G((size_t)NT * NT);
         rc = parsec_rank_gather(ctx, A, G.data(), 1);
-        if (!rc) band = hicma_band_autotune_weighted(G.data(), NT, r.NB, SERVE_LR_WEIGHT);
+        if (!rc) band = hicma_band_autotune_model(G.data(), NT, r.NB, SERVE_DENSE_TFLOPS, SERVE_LR_TFLOPS);
         if (band < 1) rc = band;
     }
     if (!rc && band > 1) {

@@ -528,6 +528,81 @@ extern "C" int hicma_band_autotune_weighted(const int *G, int NT, int NB, double
     return NT;
 }
 
+// Band choice by a time model: every factorization task of a band size b is
+// priced at the device rate of the kernel that would run it (dense tiles and
+// low-rank products on the matrix cores at dense_tflops; recompressions at
+// their reference op counts and lr_tflops, a hundredth of it for stacks
+// wider than the wave kernels' 128 columns), and the
+// b with the least total time wins.  A gemm task (m, n, k) changes kind only
+// where b passes m - n, n - k or m - k, so each task adds its time to at most
+// four ranges of b (a difference array): O(NT^3) in all.  The new rank of a
+// recompressed tile is estimated by its current rank.
+extern "C" int hicma_band_autotune_model(const int *G, int NT, int NB, double dense_tflops, double lr_tflops)
+{
+    if (!G || NT <= 0 || NB <= 0 || !(dense_tflops > 0) || !(lr_tflops > 0)) return HICMA_ERR_ARG;
+    const double nb = NB, rd = 1.0 / (dense_tflops * 1e12), rl = 1.0 / (lr_tflops * 1e12);
+    std::vector<double> D((size_t)NT + 2, 0.0);  // D[b]: time added from band b on
+    auto add = [&](int lo, int hi, double t) {     // bands lo .. hi (inclusive, clipped to 1 .. NT)
+        lo = std::max(lo, 1);
+        hi = std::min(hi, NT);
+        if (lo > hi) return;
+        D[lo] += t;
+        D[hi + 1] -= t;
+    };
+    auto rk = [&](int m, int n) { return std::max(G[(size_t)n * NT + m], 0); };
+    auto lr_case_v = [&](double ra, double rb, double rc) {
+        const unsigned long s = (unsigned long)(ra + rc), N = (unsigned long)NB, c = (unsigned long)rc;
+        const double ops = (double)(hicma_op_counts('q', N, s, 0, 0) * 2 + hicma_op_counts('m', (unsigned long)ra, (unsigned long)rb, N, 0) +
+                                    hicma_op_counts('m', (unsigned long)ra, N, (unsigned long)rb, 0) +
+                                    hicma_op_counts('r', s, s, 2, 0) + hicma_op_counts('s', s, 0, 0, 0) + s * c +
+                                    hicma_op_counts('o', N, c, s, 1) + hicma_op_counts('o', c, N, s, 2));
+        return ops;
+    };
+    for (int k = 0; k < NT; k++) {
+        for (int m = k + 1; m < NT; m++) {
+            const double r = rk(m, k), d = m - k;
+            // trsm of tile (m, k) and its syrk into (m, m): low rank while b <= d
+            add(1, (int)d, (nb * nb * r + 2 * nb * nb * r) * rd);
+            add((int)d + 1, NT, (nb * nb * nb + nb * nb * nb) * rd);
+        }
+        for (int n = k + 1; n < NT; n++)
+            for (int m = n + 1; m < NT; m++) {
+                const int x = m - n, y = n - k, z = m - k;
+                const double ra = rk(m, k), rb = rk(n, k), rc = rk(m, n);
+                const double t_all_dense = 2 * nb * nb * nb * rd;
+                const double t_cd_bd = 4 * nb * nb * ra * rd;                      // C, B dense; A low rank
+                const double t_cd = (4 * nb * ra * rb + 2 * nb * nb * rb) * rd;    // C dense; A, B low rank
+                // all low rank, and C, A low rank with B dense: stacks wider
+                // than 128 columns (or B of rank > 64) leave the wave
+                // kernels for the 256-thread one, about 100 times slower
+                const bool wide = (ra + rc > 128);
+                const double t_v = lr_case_v(ra, rb, rc) * ((wide || rb > 64) ? 100.0 * rl : rl);
+                const double t_iv = lr_case_v(ra, 0, rc) * (wide ? 100.0 * rl : rl) + 2 * nb * nb * ra * rd;
+                add(1, std::min(x, y), t_v);
+                if (x < y) {
+                    add(x + 1, y, t_cd);
+                    add(y + 1, z, t_cd_bd);
+                } else if (y < x) {
+                    add(y + 1, x, t_iv);
+                    add(x + 1, z, t_cd_bd);
+                } else {
+                    add(x + 1, z, t_cd_bd);
+                }
+                add(z + 1, NT, t_all_dense);
+            }
+    }
+    int best = 1;
+    double acc = 0.0, tbest = 0.0;
+    for (int b = 1; b <= NT; b++) {
+        acc += D[b];
+        if (b == 1 || acc < tbest) {
+            tbest = acc;
+            best = b;
+        }
+    }
+    return best;
+}
+
 extern "C" int parsec_band_size_auto_tuning(hicma_ctx *ctx, hicma_tlr_desc *Ar, hicma_tlr_desc *Fake,
                                             int *G, int NB)
 {
@@ -563,7 +638,8 @@ struct PotrfWs {
     RcWs g;
     int ns, nsf, wgrid, wres, wcap;
     bool fast, wave;
-    size_t rc_lds, fstride, winv_d, dag_d, linv_d, tasks_d, wws_d, wbs_d, wlist_d, lrd_pairs, syrk_d, lrd_d, total;
+    size_t rc_lds, fstride, winv_d, dag_d, linv_d, tasks_d, wws_d, wbs_d, wlist_d, lrd_pairs, syrk_d, lrd_d, qn_d,
+        total;
 };
 
 static void plan_potrf_ws(hicma_ctx *ctx, int NT, int NB, int cap, int b, PotrfWs &P)
@@ -607,8 +683,10 @@ static void plan_potrf_ws(hicma_ctx *ctx, int NT, int NB, int cap, int b, PotrfW
     P.wws_d = (size_t)4 * P.wgrid * 4 * wv_slot_doubles(NB);
     P.wbs_d = P.wave ? (size_t)P.wgrid * wb_slot_doubles(NB) : 0;  // one wide-stack wave per CU
     P.wlist_d = ((size_t)4 * P.wcap + 8) / 2 + 1;
+    // Qn = -B Av of the dense-B tasks the wave kernels take (2-flow only)
+    P.qn_d = (P.wave && b > 1) ? (size_t)(b - 1) * (NT - 1) * NB * QN_COLS : 0;
     P.total = g.stride * P.ns + P.syrk_d + P.lrd_d + P.fstride * P.nsf + P.tasks_d + P.winv_d + P.wws_d +
-              P.wlist_d + P.wbs_d + P.dag_d + P.linv_d;
+              P.wlist_d + P.wbs_d + P.dag_d + P.linv_d + P.qn_d;
 }
 
 // Device bytes a generation + factorization of this shape allocates on the
@@ -730,7 +808,8 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     int *wlist = wcount + 8;  // counts [0..3], next-task counters [4..7]
     double *wbs = (double *)wcount + wlist_d;
     double *Linv = wbs + P.wbs_d;
-    int *dflags = (int *)(Linv + P.linv_d);
+    double *qnws = P.qn_d ? Linv + P.linv_d : nullptr;
+    int *dflags = (int *)(Linv + P.linv_d + P.qn_d);
     HIPCHK(hipFuncSetAttribute((const void *)k_potrf_dag, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)(sizeof(double) * DAG_SMEM)));
     if (!trsm_tc(NB)) return HICMA_ERR_ARG;
@@ -876,8 +955,13 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
             if (wave) {
                 hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, wcount, 8, 0);
                 const int wide_only = (ntask < ctx->wave_min_tasks) ? 1 : 0;
+                if (qnws) {
+                    const int nrt = (NB + GT - 1) / GT;
+                    hipLaunchKernelGGL(k_qn_dense, dim3(nrt * (QN_COLS / GT), (unsigned)((b - 1) * T)), dim3(BS), 0,
+                                       st, t, k, qnws);
+                }
                 hipLaunchKernelGGL(k_classify_wave, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k,
-                                   tasks_s, wlist, wcount, wcap, wide_only);
+                                   tasks_s, wlist, wcount, wcap, wide_only, qnws ? 1 : 0);
                 // fork: variants 32 and 48 on the context stream, 64 and the
                 // wide one on side streams 0 and 1
                 HIPCHK(hipEventRecord(ctx->ev_fork, st));
@@ -885,19 +969,20 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
                 int wg = (int)std::min<long long>((wgrid - wres) * WvCfg<32>::BPC, (ntask + WvCfg<32>::WPB - 1) / WvCfg<32>::WPB);
                 hipLaunchKernelGGL(k_rc_wave<32>, dim3(wg), dim3(64 * WvCfg<32>::WPB), wlds32, st, t, k,
                                    (const int *)wlist, (const int *)wcount, wcount + 4, tasks_s, wws, acc, fixed_rank,
-                                   maxrank, o, ctx->d_ovf);
+                                   maxrank, o, ctx->d_ovf, (const double *)qnws);
                 wg = (int)std::min<long long>((wgrid - wres) * WvCfg<48>::BPC, (ntask + WvCfg<48>::WPB - 1) / WvCfg<48>::WPB);
                 hipLaunchKernelGGL(k_rc_wave<48>, dim3(wg), dim3(64 * WvCfg<48>::WPB), wlds48, st, t, k,
                                    (const int *)(wlist + wcap), (const int *)(wcount + 1), wcount + 5, tasks_s,
-                                   wws + wws_d / 4, acc, fixed_rank, maxrank, o, ctx->d_ovf);
+                                   wws + wws_d / 4, acc, fixed_rank, maxrank, o, ctx->d_ovf, (const double *)qnws);
                 wg = (int)std::min<long long>((wgrid - wres) * WvCfg<64>::BPC, (ntask + WvCfg<64>::WPB - 1) / WvCfg<64>::WPB);
                 hipLaunchKernelGGL(k_rc_wave<64>, dim3(wg), dim3(64 * WvCfg<64>::WPB), wlds64, ctx->side[0], t, k,
                                    (const int *)(wlist + 2 * (size_t)wcap), (const int *)(wcount + 2), wcount + 6,
-                                   tasks_s, wws + wws_d / 2, acc, fixed_rank, maxrank, o0, ctx->d_ovf);
+                                   tasks_s, wws + wws_d / 2, acc, fixed_rank, maxrank, o0, ctx->d_ovf,
+                                   (const double *)qnws);
                 wg = (int)std::min<long long>(wgrid - wres, ntask);
                 hipLaunchKernelGGL(k_rc_wide, dim3(wg), dim3(64 * WB_WAVES), sizeof(double) * WB_LDS, ctx->side[1], t, k,
                                    (const int *)(wlist + 3 * (size_t)wcap), (const int *)(wcount + 3), wcount + 7,
-                                   tasks_s, wbs, acc, fixed_rank, maxrank, o1, ctx->d_ovf);
+                                   tasks_s, wbs, acc, fixed_rank, maxrank, o1, ctx->d_ovf, (const double *)qnws);
                 // join
                 for (int i = 0; i < 2; i++) {
                     HIPCHK(hipEventRecord(ctx->ev_join[i], ctx->side[i]));

@@ -384,6 +384,34 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
     }
 }
 
+// Dense-B tasks (case iv: tile (n, k) in the band, (m, k) and (m, n) low
+// rank) get Qn = -B Av precomputed by k_qn_dense into a slot of QN_COLS
+// columns: slot (n - k - 1) T + (m - k - 1), T = NT - 1 - k.
+#define QN_COLS 128
+__host__ __device__ inline size_t qn_slot(int NT, int NB, int k, int m, int n)
+{
+    const int T = NT - 1 - k;
+    return ((size_t)(n - k - 1) * T + (size_t)(m - k - 1)) * NB * QN_COLS;
+}
+
+// Qn = -B Av for the dense-B tasks of step k that the wave kernels take
+// (ra > 0, s <= 128); grid (64x64 tiles of an NB x 128 slot, (band - 1) T).
+__global__ void __launch_bounds__(BS) k_qn_dense(TLR t, int k, double *qn)
+{
+    __shared__ double sm[BLK_GEMM_SMEM];
+    const int NB = t.NB, NT = t.NT, b = t.band_size, T = NT - 1 - k;
+    const int m = k + 1 + (int)(blockIdx.y % T), n = k + 1 + (int)(blockIdx.y / T);
+    if (m - n < b) return;
+    const int ra = *rank_at(t, m, k), rc = *rank_at(t, m, n);
+    if (ra <= 0 || rc + ra > 128) return;
+    const int nrt = (NB + GT - 1) / GT;
+    const int ti = blockIdx.x % nrt, tj = blockIdx.x / nrt;
+    if (tj * GT >= ra) return;
+    const double *Av = lr_tile(t, m, k) + (size_t)NB * ra;
+    gemm_tile<false, false>(ti * GT, tj * GT, NB, ra, NB, -1.0, band_tile(t, n, k), NB, Av, NB, 0.0,
+                            qn + qn_slot(NT, NB, k, m, n), NB, sm);
+}
+
 // Tasks of step k whose update runs on one wave: both factors low rank,
 // s = rc + ra in [1, 128], rb <= WV_SMAX, NB <= 64 * WV_KMAX.  They are
 // appended to list 0 (s, rb <= 32), 1 (<= 48), 2 (s <= WV_S2MAX) or 3 (s <= 128,
@@ -391,7 +419,8 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
 // a list is irrelevant (tasks of one step are independent).  Listed tasks are
 // marked -1 in tasks_s; every other task keeps its k_classify value.  With
 // wide_only, every listed task goes to list 3 (four waves per task).
-__global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wcount, int cap, int wide_only)
+__global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wcount, int cap, int wide_only,
+                                int qn_dense)
 {
     const int T = t.NT - 1 - k;
     const long long ntask = (long long)T * (T - 1) / 2;
@@ -400,10 +429,13 @@ __global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wco
     int hi, lo;
     tri_decode(e, hi, lo);
     const int m = k + 1 + hi, n = k + 1 + lo, b = t.band_size;
-    if (m - n < b || n - k < b || t.NB > 64 * WV_KMAX) return;
-    const int ra = *rank_at(t, m, k), rc = *rank_at(t, m, n), rb = *rank_at(t, n, k);
+    if (m - n < b || t.NB > 64 * WV_KMAX) return;
+    // a dense B (band tile) comes with its Qn = -B Av formed by k_qn_dense
+    const bool bdense = (n - k < b);
+    if (bdense && !qn_dense) return;
+    const int ra = *rank_at(t, m, k), rc = *rank_at(t, m, n), rb = bdense ? 0 : *rank_at(t, n, k);
     const int s = rc + ra;
-    if (ra > 0 && rb > 0 && s <= 128 && rb <= WV_SMAX) {
+    if (ra > 0 && (bdense || rb > 0) && s <= 128 && rb <= WV_SMAX) {
         const int l = wide_only ? 3 : ((s <= 32 && rb <= 32) ? 0 : ((s <= 48 && rb <= 48) ? 1 : (s <= WV_S2MAX ? 2 : 3)));
         tasks_s[e] = -1;
         wlist[(size_t)l * cap + atomicAdd(wcount + l, 1)] = (int)e;
@@ -413,7 +445,7 @@ __global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wco
 template <int SM>
 __global__ void __launch_bounds__(64 * WvCfg<SM>::WPB, WvCfg<SM>::MINW)
 k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *tasks_s, double *ws, double acc,
-          int fixed_rk, int maxrank, OpLog o, int *ovf)
+          int fixed_rk, int maxrank, OpLog o, int *ovf, const double *qn_dense)
 {
     using Cf = WvCfg<SM>;
     extern __shared__ double smem[];
@@ -441,15 +473,18 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
         int hi, lo;
         tri_decode(e, hi, lo);
         const int m = uni(k + 1 + hi), n = uni(k + 1 + lo);
-        const int ra = uni(*rank_at(t, m, k)), rc = uni(*rank_at(t, m, n)), rb = uni(*rank_at(t, n, k));
+        const bool bdense = (n - k < t.band_size);
+        const int ra = uni(*rank_at(t, m, k)), rc = uni(*rank_at(t, m, n)), rb = bdense ? 0 : uni(*rank_at(t, n, k));
         const int s = rc + ra;
         const double *Au = lr_tile(t, m, k), *Av = Au + (size_t)NB * ra;
-        const double *Bu = lr_tile(t, n, k), *Bv = Bu + (size_t)NB * rb;
+        const double *Bu = bdense ? Au : lr_tile(t, n, k), *Bv = Bu + (size_t)NB * rb;
         double *C = lr_tile(t, m, n);
+        // Q's second block: formed below, or precomputed for a dense B
+        const double *Qs = bdense ? qn_dense + qn_slot(NT, NB, k, m, n) : Qn;
 
         // ---- X = Av^T Bv (ra x rb) on the matrix cores: 16x16 tiles of X,
         //      k runs over the NB rows four at a time ----
-        {
+        if (!bdense) {
             const int i16 = lane & 15, k4 = lane >> 4;
             constexpr int TT = SM / 16, U = 4;
             wv_d4 xacc[TT][TT];
@@ -497,11 +532,11 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             wv_sync();
         }
         // ---- Qn = -Bu X^T (NB x ra): K = X^T, K[b][a] = Xl[b * SM + a] ----
-        wv_tall_mm<SM>(NB, rb, ra, -1.0, [&](int c) { return Bu + (size_t)c * NB; }, L, SM, Qn);
+        if (!bdense) wv_tall_mm<SM>(NB, rb, ra, -1.0, [&](int c) { return Bu + (size_t)c * NB; }, L, SM, Qn);
         wv_sync();
         // ---- R factors of P = [Cu Au] and Q = [Cv Qn] ----
         for (int side = 0; side < 2; side++) {
-            WvCols S{side ? C + (size_t)NB * rc : C, side ? (const double *)Qn : Au, rc, Wq, NB};
+            WvCols S{side ? C + (size_t)NB * rc : C, side ? Qs : Au, rc, Wq, NB};
             wv_geqrf_r<SM>(s, S, side ? Rq : Rp);
         }
         wv_sync();
@@ -702,7 +737,7 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             auto v_out = [&]() {  // Kv in LDS
                 wv_tall_mm<SM>(NB, s, nr, 1.0,
                                [&](int c) {
-                                   return c < rc ? (const double *)C + (size_t)(rc + c) * NB : Qn + (size_t)(c - rc) * NB;
+                                   return c < rc ? (const double *)C + (size_t)(rc + c) * NB : Qs + (size_t)(c - rc) * NB;
                                },
                                Vl, SM, C + (size_t)nr * NB);
                 wv_sync();
@@ -734,13 +769,16 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             rank_hist(o, NT, m, n, nr);
             unsigned long long mm = tdim(t, m), ss = s;
             unsigned long long qa = opc('q', mm, ss, 0, 0);
-            unsigned long long qb = opc('m', ra, rb, mm, 0) + opc('m', ra, mm, rb, 0) + opc('q', mm, ss, 0, 0);
+            // case iv (dense B) counts the product B Av, case v the products with B's factors
+            unsigned long long qb = bdense ? opc('q', mm, ss, 0, 0) + opc('m', ra, mm, mm, 0)
+                                           : opc('m', ra, rb, mm, 0) + opc('m', ra, mm, rb, 0) + opc('q', mm, ss, 0, 0);
             unsigned long long sv = opc('r', ss, ss, 2, 0) + opc('s', ss, 0, 0, 0) + ss * nr;
             unsigned long long nu = opc('o', mm, nr, ss, 1), nv = opc('o', nr, mm, ss, 2);
             oplog(o, NT, qa + qb + sv + nu + nv, m, n, false, false);
             atomicAdd(o.stat + 4, qa + qb + sv + nu + nv);
             atomicAdd(o.stat + 5, 1ULL);
-            unsigned long long words = 2ULL * NB * (ra + rc + nr) + 2ULL * NB * rb;
+            unsigned long long words =
+                2ULL * NB * (ra + rc + nr) + (bdense ? (unsigned long long)NB * NB : 2ULL * NB * rb);
             atomicAdd(o.stat + 6, 8ULL * words);
         }
         wv_sync();
@@ -979,7 +1017,7 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout,
 #define WB_WAVES 4
 __global__ void __launch_bounds__(64 * WB_WAVES, 1)
 k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *tasks_s, double *ws, double acc,
-          int fixed_rk, int maxrank, OpLog o, int *ovf)
+          int fixed_rk, int maxrank, OpLog o, int *ovf, const double *qn_dense)
 {
     extern __shared__ double smem[];
     __shared__ int s_idx;
@@ -1006,15 +1044,18 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
         int hi, lo;
         tri_decode(e, hi, lo);
         const int m = uni(k + 1 + hi), n = uni(k + 1 + lo);
-        const int ra = uni(*rank_at(t, m, k)), rc = uni(*rank_at(t, m, n)), rb = uni(*rank_at(t, n, k));
+        const bool bdense = (n - k < t.band_size);
+        const int ra = uni(*rank_at(t, m, k)), rc = uni(*rank_at(t, m, n)), rb = bdense ? 0 : uni(*rank_at(t, n, k));
         const int s = rc + ra;
         const double *Au = lr_tile(t, m, k), *Av = Au + (size_t)NB * ra;
-        const double *Bu = lr_tile(t, n, k), *Bv = Bu + (size_t)NB * rb;
+        const double *Bu = bdense ? Au : lr_tile(t, n, k), *Bv = Bu + (size_t)NB * rb;
         double *C = lr_tile(t, m, n);
+        // Q's second block: formed below, or precomputed for a dense B
+        const double *Qs = bdense ? qn_dense + qn_slot(NT, NB, k, m, n) : Qn;
         const int i16 = lane & 15, k4 = lane >> 4;
         double *J = L;
         // ---- X = Av^T Bv (ra <= 128, rb <= 64): 16-row band ta of X on wave ta % 4 ----
-        {
+        if (!bdense) {
             double *Xl = L;  // Xl[b * WB_SM + a]
             for (int q = tid; q < 64 * WB_SM; q += NTH) Xl[q] = 0.0;
             __syncthreads();
@@ -1054,8 +1095,9 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             __syncthreads();
         }
         // ---- Qn = -Bu X^T ----
-        wv_tall_mm<WB_SM>(NB, rb, ra, -1.0, [&](int c) { return Bu + (size_t)c * NB; }, L, WB_SM, Qn, wid,
-                          WB_WAVES);
+        if (!bdense)
+            wv_tall_mm<WB_SM>(NB, rb, ra, -1.0, [&](int c) { return Bu + (size_t)c * NB; }, L, WB_SM, Qn, wid,
+                              WB_WAVES);
         __syncthreads();
         // ---- R factors: P = [Cu Au] on waves 0 and 2 (trailing copy in Wq),
         //      Q = [Cv Qn] on waves 1 and 3 (trailing copy in Out, unused until
@@ -1063,7 +1105,7 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
         //      free until the Jacobi ----
         {
             const int side = wid & 1;
-            WvCols S{side ? C + (size_t)NB * rc : C, side ? (const double *)Qn : Au, rc, side ? Out : Wq, NB};
+            WvCols S{side ? C + (size_t)NB * rc : C, side ? Qs : Au, rc, side ? Out : Wq, NB};
             wb_geqrf_r(s, S, side ? Rq : Rp, wid >> 1, L + (size_t)side * WB_QR_SH);
         }
         __syncthreads();
@@ -1270,7 +1312,7 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
                 wv_tall_mm<WB_SM>(NB, s, nr, 1.0,
                                   [&](int c) {
                                       return c < rc ? (const double *)C + (size_t)(rc + c) * NB
-                                                    : Qn + (size_t)(c - rc) * NB;
+                                                    : Qs + (size_t)(c - rc) * NB;
                                   },
                                   Vl, WB_SM, C + (size_t)nr * NB, wid, WB_WAVES);
                 __syncthreads();
@@ -1303,13 +1345,16 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             rank_hist(o, NT, m, n, nr);
             unsigned long long mm = tdim(t, m), ss = s;
             unsigned long long qa = opc('q', mm, ss, 0, 0);
-            unsigned long long qb = opc('m', ra, rb, mm, 0) + opc('m', ra, mm, rb, 0) + opc('q', mm, ss, 0, 0);
+            // case iv (dense B) counts the product B Av, case v the products with B's factors
+            unsigned long long qb = bdense ? opc('q', mm, ss, 0, 0) + opc('m', ra, mm, mm, 0)
+                                           : opc('m', ra, rb, mm, 0) + opc('m', ra, mm, rb, 0) + opc('q', mm, ss, 0, 0);
             unsigned long long sv = opc('r', ss, ss, 2, 0) + opc('s', ss, 0, 0, 0) + ss * nr;
             unsigned long long nu = opc('o', mm, nr, ss, 1), nv = opc('o', nr, mm, ss, 2);
             oplog(o, NT, qa + qb + sv + nu + nv, m, n, false, false);
             atomicAdd(o.stat + 4, qa + qb + sv + nu + nv);
             atomicAdd(o.stat + 5, 1ULL);
-            unsigned long long words = 2ULL * NB * (ra + rc + nr) + 2ULL * NB * rb;
+            unsigned long long words =
+                2ULL * NB * (ra + rc + nr) + (bdense ? (unsigned long long)NB * NB : 2ULL * NB * rb);
             atomicAdd(o.stat + 6, 8ULL * words);
         }
         __syncthreads();

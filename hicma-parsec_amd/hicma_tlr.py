@@ -26,6 +26,7 @@ EXPORTS = [
     "hicma_init", "hicma_fini", "hicma_desc_create", "hicma_desc_destroy", "hicma_desc_info",
     "hicma_get_points", "hicma_points", "STARSH_gen", "parsec_rank_gather", "parsec_rank_check",
     "HICMA_get_stat2", "parsec_band_size_auto_tuning", "hicma_band_autotune", "hicma_band_autotune_weighted",
+    "hicma_band_autotune_model",
     "parsec_band_regenerate",
     "HiCMA_dpotrf_L", "hicma_op_counts", "hicma_op_stats", "hicma_rank_overflows", "STARSH_check",
     "hicma_get_tile", "hicma_set_tile", "hicma_get_rank_map", "hicma_logdet", "hicma_recompress_stats",
@@ -80,6 +81,7 @@ def lib():
             "parsec_band_size_auto_tuning": (i, [vp, vp, vp, ip, i]),
             "hicma_band_autotune": (i, [ip, i, i]),
             "hicma_band_autotune_weighted": (i, [ip, i, i, d]),
+            "hicma_band_autotune_model": (i, [ip, i, i, d, d]),
             "parsec_band_regenerate": (i, [vp, i, vp, vp, vp, vp, i]),
             "HiCMA_dpotrf_L": (i, [vp, i, vp, vp, vp, d, i, i, C.POINTER(i), i, i, i, i, C.POINTER(i),
                                    vp, vp, vp, i]),
@@ -161,8 +163,15 @@ def band_autotune(G, NT, NB, lr_weight=None):
     return lib().hicma_band_autotune_weighted(G.ctypes.data, NT, NB, float(lr_weight))
 
 
-# weight of the low-rank op counts in the serving loop's band choice
-SERVE_LR_WEIGHT = 8.0
+def band_autotune_model(G, NT, NB, dense_tflops=10.0, lr_tflops=3.5):
+    """hicma_band_autotune_model: the band with the least estimated time."""
+    G = np.ascontiguousarray(G, dtype=np.int32)
+    return lib().hicma_band_autotune_model(G.ctypes.data, NT, NB, float(dense_tflops), float(lr_tflops))
+
+
+# device rates of the serving loop's band model (TFLOP/s)
+SERVE_DENSE_TFLOPS = 10.0
+SERVE_LR_TFLOPS = 3.5
 
 
 def rank_stat(G, NT, band):

@@ -115,6 +115,13 @@ int hicma_band_autotune(const int *rank_array, int NT, int NB);
  * gemms run on the matrix cores many times faster per op than the low-rank
  * recompressions.  No counterpart in the reference. */
 int hicma_band_autotune_weighted(const int *rank_array, int NT, int NB, double lr_weight);
+
+/* Band choice by a time model of this device: the band size with the least
+ * estimated factorization time, dense work priced at dense_tflops and the
+ * recompressions (the reference's op counts) at lr_tflops (a hundredth of it
+ * for stacks wider than 128 columns).  No counterpart in the reference; the
+ * serving loop uses it. */
+int hicma_band_autotune_model(const int *rank_array, int NT, int NB, double dense_tflops, double lr_tflops);
 /* parsec_band_regenerate (hicma_parsec/hicma_parsec.h:108-121): dense band
  * tiles regenerated in HBM; data / kernel / index as for STARSH_gen */
 int parsec_band_regenerate(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, void *data, hicma_kernel *kernel,

@@ -39,7 +39,8 @@ def test_alone_equals_mix_equals_direct():
         for a, (N, s2, b, sd) in zip(alone[:3], REQS[:3]):
             g = H.TLRMatrix(ctx, N, 512, 256, H.SQEXP_2D, s2, b, 1e-2, sd)
             assert g.generate(1e-8, 256, 1) == 0
-            band = H.band_autotune(g.rank_gather(), g.NT, g.NB, H.SERVE_LR_WEIGHT)  # the server's band choice
+            band = H.band_autotune_model(g.rank_gather(), g.NT, g.NB, H.SERVE_DENSE_TFLOPS,
+                                         H.SERVE_LR_TFLOPS)  # the server's band choice
             if band != 1:
                 g.regenerate(band)
             assert g.potrf(1e-8, 0, 256) == 0

@@ -104,3 +104,55 @@ def test_weighted_band_autotune():
                 prev = bw
     full = np.full(100, 20, np.int32)
     assert H.band_autotune(full, 10, 270, 1e9) == 10
+
+
+def _model_time_bruteforce(G, NT, NB, b, rd, rl):
+    """The time model of hicma_band_autotune_model evaluated task by task
+    (O(NT^3) per band): the reference it is checked against."""
+    rk = lambda m, n: max(int(G[n * NT + m]), 0)
+    nb = float(NB)
+    rd, rl = 1.0 / (rd * 1e12), 1.0 / (rl * 1e12)
+
+    def case_v(ra, rb, rc):
+        s = ra + rc
+        return (2 * H.op_counts('q', NB, s) + H.op_counts('m', ra, rb, NB) + H.op_counts('m', ra, NB, rb) +
+                H.op_counts('r', s, s, 2) + H.op_counts('s', s) + s * rc + H.op_counts('o', NB, rc, s, 1) +
+                H.op_counts('o', rc, NB, s, 2))
+    t = 0.0
+    for k in range(NT):
+        for m in range(k + 1, NT):
+            r = rk(m, k)
+            t += (3 * nb * nb * r if m - k >= b else 2 * nb ** 3) * rd
+        for n in range(k + 1, NT):
+            for m in range(n + 1, NT):
+                ra, rb, rc = rk(m, k), rk(n, k), rk(m, n)
+                dA, dB, dC = m - k < b, n - k < b, m - n < b
+                wide = ra + rc > 128
+                if dA:
+                    t += 2 * nb ** 3 * rd
+                elif dC and dB:
+                    t += 4 * nb * nb * ra * rd
+                elif dC:
+                    t += (4 * nb * ra * rb + 2 * nb * nb * rb) * rd
+                elif dB:
+                    t += case_v(ra, 0, rc) * (100 * rl if wide else rl) + 2 * nb * nb * ra * rd
+                else:
+                    t += case_v(ra, rb, rc) * (100 * rl if (wide or rb > 64) else rl)
+    return t
+
+
+def test_band_model_equals_bruteforce():
+    """hicma_band_autotune_model's difference-array evaluation picks the
+    band that a direct task-by-task evaluation of the same model picks."""
+    rng = np.random.default_rng(3)
+    for NT, NB in [(6, 270), (9, 512), (12, 900)]:
+        for scale in (5, 60, 200):
+            G = np.zeros((NT, NT), np.int32)
+            for m in range(NT):
+                for n in range(m):
+                    G[n, m] = int(rng.integers(0, scale + 1) * np.exp(-(m - n) / 3.0))
+            Gm = G.ravel()
+            times = [_model_time_bruteforce(Gm, NT, NB, b, 10.0, 3.5) for b in range(1, NT + 1)]
+            want = 1 + int(np.argmin(times))
+            assert H.band_autotune_model(Gm, NT, NB, 10.0, 3.5) == want, (NT, NB, scale)
+    assert H.band_autotune_model(np.zeros(100, np.int32), 10, 270) == 1
