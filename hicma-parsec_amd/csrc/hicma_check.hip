@@ -169,8 +169,8 @@ static int sum_parts(hicma_ctx *c, const double *part, long n, double *out)
 {
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, c->stream, part, n, c->d_scalar);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(out, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -281,8 +281,8 @@ extern "C" int hicma_dense_potrf(hicma_ctx *ctx, int uplo, hicma_dense *D, int *
         return st == rocblas_status_memory_error ? HICMA_ERR_NOMEM : HICMA_ERR_HIP;
     }
     int64_t hinfo = 0;
-    HIPCHK(hipMemcpyAsync(&hinfo, dinfo, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(&hinfo, dinfo, sizeof(int64_t), hipMemcpyDeviceToHost));
     if (info) *info = (int)hinfo;
     return 0;
 }

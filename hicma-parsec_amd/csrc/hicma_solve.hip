@@ -393,8 +393,11 @@ extern "C" int hicma_tlr_potrs(hicma_ctx *ctx, hicma_tlr_desc *A, int nrhs, doub
 int loglik_finish(hicma_ctx *ctx, const TLR &t, double *d_vals, double *logdet, double *quad, double *loglik)
 {
     double h[2];
-    HIPCHK(hipMemcpyAsync(h, d_vals, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    // wait for the stream first, then copy: a copy into pageable memory that
+    // waits for its stream holds the runtime's staging path meanwhile, and
+    // other threads' copies queue behind it
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(h, d_vals, 2 * sizeof(double), hipMemcpyDeviceToHost));
     if (quad) *quad = h[0];
     if (logdet) *logdet = h[1];
     if (loglik) *loglik = -0.5 * (h[0] + h[1] + (double)t.N * log(2.0 * 3.14159265358979323846));

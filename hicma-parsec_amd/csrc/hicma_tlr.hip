@@ -139,11 +139,16 @@ extern "C" int hicma_init_ex(hicma_ctx **ctx, int device, int flags, int cu_shar
     }
     HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     if (c->pool) {
-        // keep freed blocks in the device's pool instead of returning them
+        // keep freed blocks in the device's pool instead of returning them,
+        // and never let one context's allocation reuse another stream's
+        // freed block by making its stream wait for that one (the contexts
+        // of a server would then serialize)
         hipMemPool_t mp;
         HIPCHK(hipDeviceGetDefaultMemPool(&mp, device));
         uint64_t thr = ~0ULL;
         HIPCHK(hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &thr));
+        int no = 0;
+        HIPCHK(hipMemPoolSetAttribute(mp, hipMemPoolReuseAllowInternalDependencies, &no));
     }
     HIPCHK(hipMalloc(&c->d_info, 4 * sizeof(int)));
     HIPCHK(hipMemset(c->d_info, 0, 4 * sizeof(int)));
@@ -395,7 +400,7 @@ extern "C" int STARSH_gen(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr
         sketch(NB, p, SKETCH_SEED, om.data());
         double *d_om = nullptr;
         HIPCHK(ctx_malloc(ctx, (void **)&d_om, sizeof(double) * om.size()));
-        HIPCHK(hipMemcpyAsync(d_om, om.data(), sizeof(double) * om.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpy(d_om, om.data(), sizeof(double) * om.size(), hipMemcpyHostToDevice));  // stream idle (ctx_malloc)
         GenWs g;
         g.p = p;
         g.stride = gen_slot_doubles(NB, p, t.cap);
@@ -1294,7 +1299,7 @@ extern "C" int hicma_logdet(hicma_tlr_desc *A, double *logdet)
     HIPCHK(hipSetDevice(c->device));
     hipLaunchKernelGGL(k_logdet, dim3(1), dim3(BS), 0, c->stream, A->t, c->d_scalar);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(logdet, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(logdet, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
