@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # NB=900, maxrank 150, eps 1e-8), measured with two rocprofv3 --pmc passes
 # (FETCH_SIZE, WRITE_SIZE) summed over the group's kernels and scaled by the
 # counters' reading of a device copy of known size (x2.0 and x1.0).
-C2_TRAFFIC_BYTES_PER_GROUP = 6.873e9
+C2_TRAFFIC_BYTES_PER_GROUP = 6.559e9
 
 
 def log(*a):
@@ -193,6 +193,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--schedule", type=int, default=None,
+                    help="fixed schedule threshold (tasks per step below which the four-wave kernel runs)")
     ap.add_argument("--serve-rates", default="2,8,32,128")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -219,7 +221,11 @@ def main():
     pristine = H.TLRMatrix(ctx, a.N, a.NB, a.maxrank, H.SQEXP_2D, 1.0, 0.1, 1e-2, 0)
     pristine.copy_from(A)
     sched = 0
-    if not a.no_autotune:
+    if a.schedule is not None:
+        sched = a.schedule
+        ctx.set_wave_min_tasks(sched)
+        log(f"schedule: four-wave kernel below {sched} tasks per step (given)")
+    elif not a.no_autotune:
         t0 = time.time()
         sched, ts = A.autotune_schedule(a.eps, a.maxrank, reps=2)
         log(f"schedule autotuner: four-wave kernel below {sched} tasks per step ({ts:.3f}s; {time.time() - t0:.1f}s)")
