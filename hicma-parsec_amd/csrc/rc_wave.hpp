@@ -190,28 +190,116 @@ struct WvCols {
     }
 };
 
+// Buffer resources of the stack's columns for wv_geqrf_r (the fields as plain
+// scalars: no indexed access to a structure in private memory).
+struct WvRs {
+    const double *src0, *src1, *W;
+    int c0, NB;
+    // column c (no bytes when !valid: loads give 0)
+    __device__ __forceinline__ wv_rsrc rd(int c, bool first, bool valid) const
+    {
+        const double *p = !first ? W + (size_t)c * NB
+                                 : (c < c0 ? src0 + (size_t)c * NB : src1 + (size_t)(c - c0) * NB);
+        return wv_col_rsrc(valid ? p : W, valid ? NB * 8 : 0);
+    }
+    // W's column c (no bytes when !valid: stores are dropped)
+    __device__ __forceinline__ wv_rsrc wr(int c, bool valid) const
+    {
+        return wv_col_rsrc(W + (size_t)(valid ? c : 0) * NB, valid ? NB * 8 : 0);
+    }
+};
+
+// One trailing pair (b0, b1) of a panel: b -= V (T^T (V^T b)), V = the
+// panel's reflectors (row block 0 in v0, the rest in a), result to W.
+__device__ __forceinline__ void wv_qr_pair(const double (&a)[WV_KMAX][WV_PW], const double (&v0)[WV_PW],
+                                           const double (&T)[WV_PW][WV_PW], double (&b0)[WV_KMAX],
+                                           double (&b1)[WV_KMAX], wv_rsrc w0, wv_rsrc w1, int lane)
+{
+    double p[2 * WV_PW];
+#pragma unroll
+    for (int c = 0; c < WV_PW; c++) {
+        double a0 = v0[c] * b0[0], a1 = v0[c] * b1[0];
+#pragma unroll
+        for (int k = 1; k < WV_KMAX; k++) {
+            a0 = fma(a[k][c], b0[k], a0);
+            a1 = fma(a[k][c], b1[k], a1);
+        }
+        p[c] = a0;
+        p[c + WV_PW] = a1;
+    }
+    const double red = wrsN<2 * WV_PW>(p);
+    double w0v[WV_PW], w1v[WV_PW];
+    {
+        double d0[WV_PW], d1[WV_PW];
+#pragma unroll
+        for (int c = 0; c < WV_PW; c++) {
+            d0[c] = rdlane(red, (32 / WV_PW) * c);
+            d1[c] = rdlane(red, (32 / WV_PW) * (c + WV_PW));
+        }
+#pragma unroll
+        for (int c = 0; c < WV_PW; c++) {
+            double x0 = 0.0, x1 = 0.0;
+#pragma unroll
+            for (int q = 0; q <= c; q++) {
+                x0 = fma(T[q][c], d0[q], x0);
+                x1 = fma(T[q][c], d1[q], x1);
+            }
+            w0v[c] = x0;
+            w1v[c] = x1;
+        }
+    }
+    {
+        double x0 = b0[0], x1 = b1[0];
+#pragma unroll
+        for (int c = 0; c < WV_PW; c++) {
+            x0 = fma(-v0[c], w0v[c], x0);
+            x1 = fma(-v0[c], w1v[c], x1);
+        }
+        b0[0] = x0;
+        b1[0] = x1;
+    }
+#pragma unroll
+    for (int k = 1; k < WV_KMAX; k++) {
+        double x0 = b0[k], x1 = b1[k];
+#pragma unroll
+        for (int c = 0; c < WV_PW; c++) {
+            x0 = fma(-a[k][c], w0v[c], x0);
+            x1 = fma(-a[k][c], w1v[c], x1);
+        }
+        b0[k] = x0;
+        b1[k] = x1;
+    }
+    wv_st_col(b0, w0, lane);
+    wv_st_col(b1, w1, lane);
+}
+
 // R factor of the NB x s stack (s <= SM <= 64, so pivot row j is on lane
 // j of the first row block).  Panels of WV_PW columns live in registers,
-// trailing columns stream through W.  R (row-major, ld WV_SMAX, zeros below
-// the diagonal) is written to Rout.
+// trailing columns stream through W in pairs: the pair after the current
+// one is loaded while the current one is updated (two register sets used in
+// turn, so no load is waited for before its use), and a panel's first pair
+// is loaded before the panel is factored.  R (row-major, ld WV_SMAX, zeros
+// below the diagonal) is written to Rout.
 template <int SM>
-__device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
+__device__ __forceinline__ void wv_geqrf_r(int s, const double *src0, const double *src1, int c0, double *W, int NB,
+                                           double *Rout)
 {
     const int lane = threadIdx.x & 63;
-    const int NB = A.NB;
+    const WvRs A{src0, src1, W, c0, NB};
     for (int j0 = 0; j0 < s; j0 += WV_PW) {
         const int jb = min(WV_PW, s - j0);
         const bool first = (j0 == 0);
         double a[WV_KMAX][WV_PW];
 #pragma unroll
         for (int c = 0; c < WV_PW; c++) {
-            const double *col = A.col(j0 + c, first);
+            const wv_rsrc r = A.rd(j0 + c, first, c < jb);
 #pragma unroll
-            for (int k = 0; k < WV_KMAX; k++) {
-                const int i = lane + 64 * k;
-                a[k][c] = (c < jb && i < NB) ? col[i] : 0.0;
-            }
+            for (int k = 0; k < WV_KMAX; k++) a[k][c] = wv_ld(r, (lane + 64 * k) * 8);
         }
+        const int t0 = j0 + jb;
+        double p0[WV_KMAX], p1[WV_KMAX], q0[WV_KMAX], q1[WV_KMAX];
+        wv_ld_col(p0, A.rd(t0, first, t0 < s), lane);
+        wv_ld_col(p1, A.rd(t0 + 1, first, t0 + 1 < s), lane);
         double T[WV_PW][WV_PW];
 #pragma unroll
         for (int r = 0; r < WV_PW; r++)
@@ -283,103 +371,16 @@ __device__ __forceinline__ void wv_geqrf_r(int s, const WvCols &A, double *Rout)
 #pragma unroll
         for (int c = 0; c < WV_PW; c++)
             v0[c] = (c < jb) ? ((lane == j0 + c) ? 1.0 : (lane > j0 + c ? a[0][c] : 0.0)) : 0.0;
-        // ---- trailing columns, two at a time: b -= V (T^T (V^T b)) ----
-        // the next pair is loaded before the current one is processed, so
-        // its memory latency overlaps the reduction and update
-        double n0[WV_KMAX], n1[WV_KMAX];
-        {
-            const int t = j0 + jb;
-            const bool two = (t + 1 < s);
-            const double *col0 = A.col(t < s ? t : j0, first), *col1 = A.col(two ? t + 1 : j0, first);
-#pragma unroll
-            for (int k = 0; k < WV_KMAX; k++) {
-                const int i = lane + 64 * k;
-                n0[k] = (t < s && i < NB) ? col0[i] : 0.0;
-                n1[k] = (two && i < NB) ? col1[i] : 0.0;
-            }
-        }
-        for (int t = j0 + jb; t < s; t += 2) {
-            const bool two = (t + 1 < s);
-            double b0[WV_KMAX], b1[WV_KMAX];
-#pragma unroll
-            for (int k = 0; k < WV_KMAX; k++) {
-                b0[k] = n0[k];
-                b1[k] = n1[k];
-            }
-            {
-                const int tn = t + 2;
-                const bool nx0 = tn < s, nx1 = tn + 1 < s;
-                const double *col0 = A.col(nx0 ? tn : t, first), *col1 = A.col(nx1 ? tn + 1 : t, first);
-#pragma unroll
-                for (int k = 0; k < WV_KMAX; k++) {
-                    const int i = lane + 64 * k;
-                    n0[k] = (nx0 && i < NB) ? col0[i] : 0.0;
-                    n1[k] = (nx1 && i < NB) ? col1[i] : 0.0;
-                }
-            }
-            double p[2 * WV_PW];
-#pragma unroll
-            for (int c = 0; c < WV_PW; c++) {
-                double a0 = v0[c] * b0[0], a1 = v0[c] * b1[0];
-#pragma unroll
-                for (int k = 1; k < WV_KMAX; k++) {
-                    a0 = fma(a[k][c], b0[k], a0);
-                    a1 = fma(a[k][c], b1[k], a1);
-                }
-                p[c] = a0;
-                p[c + WV_PW] = a1;
-            }
-            const double red = wrsN<2 * WV_PW>(p);
-            double w0[WV_PW], w1[WV_PW];
-            {
-                double d0[WV_PW], d1[WV_PW];
-#pragma unroll
-                for (int c = 0; c < WV_PW; c++) {
-                    d0[c] = rdlane(red, (32 / WV_PW) * c);
-                    d1[c] = rdlane(red, (32 / WV_PW) * (c + WV_PW));
-                }
-#pragma unroll
-                for (int c = 0; c < WV_PW; c++) {
-                    double x0 = 0.0, x1 = 0.0;
-#pragma unroll
-                    for (int q = 0; q <= c; q++) {
-                        x0 = fma(T[q][c], d0[q], x0);
-                        x1 = fma(T[q][c], d1[q], x1);
-                    }
-                    w0[c] = x0;
-                    w1[c] = x1;
-                }
-            }
-            {
-                double x0 = b0[0], x1 = b1[0];
-#pragma unroll
-                for (int c = 0; c < WV_PW; c++) {
-                    x0 = fma(-v0[c], w0[c], x0);
-                    x1 = fma(-v0[c], w1[c], x1);
-                }
-                b0[0] = x0;
-                b1[0] = x1;
-            }
-#pragma unroll
-            for (int k = 1; k < WV_KMAX; k++) {
-                double x0 = b0[k], x1 = b1[k];
-#pragma unroll
-                for (int c = 0; c < WV_PW; c++) {
-                    x0 = fma(-a[k][c], w0[c], x0);
-                    x1 = fma(-a[k][c], w1[c], x1);
-                }
-                b0[k] = x0;
-                b1[k] = x1;
-            }
-            double *out0 = A.W + (size_t)t * NB, *out1 = A.W + (size_t)(t + 1) * NB;
-#pragma unroll
-            for (int k = 0; k < WV_KMAX; k++) {
-                const int i = lane + 64 * k;
-                if (i < NB) {
-                    out0[i] = b0[k];
-                    if (two) out1[i] = b1[k];
-                }
-            }
+        // ---- trailing pairs: (t, t + 1) in p while (t + 2, t + 3) loads into q,
+        //      then the roles swap ----
+        for (int t = t0; t < s; t += 4) {
+            wv_ld_col(q0, A.rd(t + 2, first, t + 2 < s), lane);
+            wv_ld_col(q1, A.rd(t + 3, first, t + 3 < s), lane);
+            wv_qr_pair(a, v0, T, p0, p1, A.wr(t, true), A.wr(t + 1, t + 1 < s), lane);
+            if (t + 2 >= s) break;
+            wv_ld_col(p0, A.rd(t + 4, first, t + 4 < s), lane);
+            wv_ld_col(p1, A.rd(t + 5, first, t + 5 < s), lane);
+            wv_qr_pair(a, v0, T, q0, q1, A.wr(t + 2, true), A.wr(t + 3, t + 3 < s), lane);
         }
     }
 }
@@ -536,8 +537,7 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
         wv_sync();
         // ---- R factors of P = [Cu Au] and Q = [Cv Qn] ----
         for (int side = 0; side < 2; side++) {
-            WvCols S{side ? C + (size_t)NB * rc : C, side ? Qs : Au, rc, Wq, NB};
-            wv_geqrf_r<SM>(s, S, side ? Rq : Rp);
+            wv_geqrf_r<SM>(s, side ? C + (size_t)NB * rc : C, side ? Qs : Au, rc, Wq, NB, side ? Rq : Rp);
         }
         wv_sync();
         // ---- rows of M = Rp Rq^T: lane c holds M(c, :) ----

@@ -124,3 +124,43 @@ __device__ __forceinline__ void wv_sync()
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+// Raw buffer access to one column of a tile or workspace.  The resource
+// (gfx9 word 3 = 0x00020000: untyped, unswizzled) bounds the column: a load
+// at or past `bytes` returns 0 and a store there is dropped, so rows past the
+// tile and columns given 0 bytes need no branch.  Buffer loads count only
+// against vmcnt and are issued from a scalar base, so loads issued ahead of
+// use are waited for at their use, not at the next LDS or scalar access.
+typedef unsigned int wv_u2 __attribute__((ext_vector_type(2)));
+typedef __amdgpu_buffer_rsrc_t wv_rsrc;
+__device__ __forceinline__ wv_rsrc wv_col_rsrc(const double *p, int bytes)
+{
+    // the base and size are wave-uniform; readfirstlane keeps the resource in
+    // scalar registers whatever the compiler can prove
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    double *u = (double *)(((unsigned long long)hi << 32) | lo);
+    return __builtin_amdgcn_make_buffer_rsrc(u, 0, __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+}
+__device__ __forceinline__ double wv_ld(wv_rsrc r, int off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void wv_st(double v, wv_rsrc r, int off)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(wv_u2, v), r, off, 0, 0);
+}
+// rows lane, lane + 64, ... of a column (KR rows per lane)
+template <int KR>
+__device__ __forceinline__ void wv_ld_col(double (&x)[KR], wv_rsrc r, int lane)
+{
+#pragma unroll
+    for (int k = 0; k < KR; k++) x[k] = wv_ld(r, (lane + 64 * k) * 8);
+}
+template <int KR>
+__device__ __forceinline__ void wv_st_col(const double (&x)[KR], wv_rsrc r, int lane)
+{
+#pragma unroll
+    for (int k = 0; k < KR; k++) wv_st(x[k], r, (lane + 64 * k) * 8);
+}
