@@ -64,62 +64,192 @@ __device__ __forceinline__ wv_d4 mfma16(double a, double b, wv_d4 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// Tall product D (NB x nc, ld NB) = alpha * A (NB x kd) K (kd x nc) with A's
-// column c at acol(c) and K in LDS (K[c][j] at Kl[c * ldk + j], zero for
-// j >= nc).  RT 16-row tiles at a time (tiles t0, t0 + tstep, ... when several
-// waves share the product), all their A loads issued before the first
-// MFMA, so a wave keeps RT tiles of loads in flight; the K operands are read
-// from LDS.  Rows of a batch are read before any of them is written, so D
-// may overwrite A's storage row for row.
-template <int SM, int RT = (SM <= 32 ? 4 : (SM <= 64 ? 2 : 1)), typename ACol>
-__device__ __forceinline__ void wv_tall_mm(int NB, int kd, int nc, double alpha, ACol acol, const double *Kl,
-                                           int ldk, double *D, int t0 = 0, int tstep = 1)
+// Columns of a tall operand: c0 columns at s0, the rest at s1, column
+// stride lda (rows 0 .. NB - 1 of each are used).
+struct WvSrc {
+    const double *s0;
+    int c0;
+    const double *s1;
+    int lda;
+};
+
+// One batch of A rows by buffer loads.  Column group kk (columns 4 kk ..
+// 4 kk + 3, one per 16 lanes) is read from the block that holds it; when c0
+// is not a multiple of 4 the group that straddles the blocks reads its
+// block-0 columns into av and its block-1 columns into ax, and the MFMA
+// chain takes both.  A lane outside its block reads 0, so there is neither
+// a branch nor a select after a load, and loads are waited for at their use.
+template <int SM, int RT>
+__device__ __forceinline__ void wv_tall_ld(double (&av)[RT][SM / 4], double (&ax)[RT], int NB, int kd, wv_rsrc r0s,
+                                           wv_rsrc r1s, int c0, int lda, int r0, int rstep)
+{
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+    const int cx = 4 * (c0 >> 2) + k4;  // this lane's column of the straddling group
+    const bool mix = (c0 & 3) != 0 && cx >= c0 && cx < kd;
+#pragma unroll
+    for (int t = 0; t < RT; t++) {
+        const int row = r0 + rstep * t + i16;
+#pragma unroll
+        for (int kk = 0; kk < SM / 4; kk++) {
+            const int c = 4 * kk + k4;
+            const bool b1 = 4 * kk >= c0;  // the whole group is in block 1
+            const bool in = row < NB && c < kd && (b1 || c < c0);
+            av[t][kk] = wv_ld(b1 ? r1s : r0s, in ? ((b1 ? c - c0 : c) * lda + row) * 8 : WV_OOB);
+        }
+        ax[t] = wv_ld(r1s, (mix && row < NB) ? ((cx - c0) * lda + row) * 8 : WV_OOB);
+    }
+}
+
+// This lane's K operands: kv[kk][tn] = K[4 kk + k4][16 tn + i16] and
+// kx[tn] = the same for the straddling group's block-1 columns, 0 where
+// there is no such row.
+template <int SM>
+__device__ __forceinline__ void wv_tall_k(double (&kv)[SM / 4][SM / 16], double (&kx)[SM / 16], int kd, int c0,
+                                          const double *Kl, int ldk)
+{
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < SM / 4; kk++)
+#pragma unroll
+        for (int tn = 0; tn < SM / 16; tn++) {
+            const int kr = 4 * kk + k4;
+            const double v = Kl[(kr < kd ? kr : 0) * ldk + 16 * tn + i16];
+            kv[kk][tn] = kr < kd ? v : 0.0;
+        }
+    const int cx = 4 * (c0 >> 2) + k4;
+    const bool mix = (c0 & 3) != 0 && cx >= c0 && cx < kd;
+#pragma unroll
+    for (int tn = 0; tn < SM / 16; tn++) {
+        const double v = Kl[(mix ? cx : 0) * ldk + 16 * tn + i16];
+        kx[tn] = mix ? v : 0.0;
+    }
+}
+
+// D rows of one batch from its loaded A rows and the K operands in
+// registers: for each 16-column block of D (a uniform branch on nc), the
+// MFMA chain over K runs in chunks of 16 rows (a uniform branch on kd per
+// chunk), so the accumulators stay in place between MFMAs.  Stores go
+// through a buffer over D's nc columns (those past it are dropped).
+template <int SM, int RT>
+__device__ __forceinline__ void wv_tall_mul(const double (&av)[RT][SM / 4], const double (&ax)[RT],
+                                            const double (&kv)[SM / 4][SM / 16], const double (&kx)[SM / 16], int NB,
+                                            int kd, int nc, double alpha, wv_rsrc dr, int r0, int rstep)
 {
     const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
     constexpr int KS = SM / 4, TN = SM / 16;
-#pragma unroll 1
-    for (int r0 = 16 * t0; r0 < NB; r0 += 16 * tstep * RT) {  // batches of RT row tiles
-        double av[RT][KS];
 #pragma unroll
-        for (int t = 0; t < RT; t++) {
-            const int row = r0 + 16 * tstep * t + i16;
+    for (int tn = 0; tn < TN; tn++) {
+        if (16 * tn < nc) {
+            wv_d4 acc[RT];
 #pragma unroll
-            for (int kk = 0; kk < KS; kk++) {
-                const int c = 4 * kk + k4;
-                av[t][kk] = (4 * kk < kd && c < kd && row < NB) ? acol(c)[row] : 0.0;
-            }
-        }
-        wv_d4 acc[RT][TN];
+            for (int t = 0; t < RT; t++) acc[t] = mfma16(ax[t], kx[tn], wv_d4{0.0, 0.0, 0.0, 0.0});
 #pragma unroll
-        for (int t = 0; t < RT; t++)
+            for (int kc = 0; kc < KS; kc += 4) {
+                if (4 * kc < kd) {
 #pragma unroll
-            for (int tn = 0; tn < TN; tn++) acc[t][tn] = wv_d4{0.0, 0.0, 0.0, 0.0};
+                    for (int kk = kc; kk < kc + 4; kk++)
 #pragma unroll
-        for (int kk = 0; kk < KS; kk++)
-            if (4 * kk < kd) {
-                const int kr = 4 * kk + k4;
-#pragma unroll
-                for (int tn = 0; tn < TN; tn++)
-                    if (16 * tn < nc) {
-                        const double kb = (kr < kd) ? Kl[kr * ldk + 16 * tn + i16] : 0.0;
-#pragma unroll
-                        for (int t = 0; t < RT; t++) acc[t][tn] = mfma16(av[t][kk], kb, acc[t][tn]);
-                    }
-            }
-#pragma unroll
-        for (int t = 0; t < RT; t++) {
-            const int rt = r0 + 16 * tstep * t;
-#pragma unroll
-            for (int tn = 0; tn < TN; tn++) {
-                if (16 * tn < nc) {
-                    const int j = 16 * tn + i16;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int i = rt + k4 + 4 * r;
-                        if (j < nc && i < NB) D[i + (size_t)j * NB] = alpha * acc[t][tn][r];
-                    }
+                        for (int t = 0; t < RT; t++) acc[t] = mfma16(av[t][kk], kv[kk][tn], acc[t]);
                 }
             }
+            const int j = 16 * tn + i16;
+#pragma unroll
+            for (int t = 0; t < RT; t++) {
+                const int rt = r0 + rstep * t;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = rt + k4 + 4 * r;
+                    wv_st(alpha * acc[t][r], dr, (j < nc && i < NB) ? (i + j * NB) * 8 : WV_OOB);
+                }
+            }
+        }
+    }
+}
+
+// The same with K read from LDS, a chunk of 16 K rows at a time (for the
+// widest stacks, whose K does not fit in registers).
+template <int SM, int RT>
+__device__ __forceinline__ void wv_tall_mul_lds(const double (&av)[RT][SM / 4], const double (&ax)[RT], int NB, int kd,
+                                                int c0, int nc, double alpha, const double *Kl, int ldk, wv_rsrc dr,
+                                                int r0, int rstep)
+{
+    const int lane = threadIdx.x & 63, i16 = lane & 15, k4 = lane >> 4;
+    constexpr int KS = SM / 4, TN = SM / 16;
+    const int cx = 4 * (c0 >> 2) + k4;
+    const bool mix = (c0 & 3) != 0 && cx >= c0 && cx < kd;
+#pragma unroll
+    for (int tn = 0; tn < TN; tn++) {
+        if (16 * tn < nc) {
+            const double kbx = Kl[(mix ? cx : 0) * ldk + 16 * tn + i16];
+            wv_d4 acc[RT];
+#pragma unroll
+            for (int t = 0; t < RT; t++) acc[t] = mfma16(ax[t], mix ? kbx : 0.0, wv_d4{0.0, 0.0, 0.0, 0.0});
+#pragma unroll
+            for (int kc = 0; kc < KS; kc += 4) {
+                if (4 * kc < kd) {
+                    double kb[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int kr = 4 * (kc + u) + k4;
+                        const double v = Kl[(kr < kd ? kr : 0) * ldk + 16 * tn + i16];
+                        kb[u] = kr < kd ? v : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++)
+#pragma unroll
+                        for (int t = 0; t < RT; t++) acc[t] = mfma16(av[t][kc + u], kb[u], acc[t]);
+                }
+            }
+            const int j = 16 * tn + i16;
+#pragma unroll
+            for (int t = 0; t < RT; t++) {
+                const int rt = r0 + rstep * t;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = rt + k4 + 4 * r;
+                    wv_st(alpha * acc[t][r], dr, (j < nc && i < NB) ? (i + j * NB) * 8 : WV_OOB);
+                }
+            }
+        }
+    }
+}
+
+// Tall product D (NB x nc, ld NB) = alpha * A (NB x kd) K (kd x nc) with A's
+// columns given by a WvSrc and K in LDS (K[c][j] at Kl[c * ldk + j]).  Work
+// goes in batches of RT 16-row tiles (tiles t0, t0 + tstep, ... when several
+// waves share the product); a batch's A loads are all issued before its
+// first MFMA.  With PP, K is held in registers and batches alternate
+// between two register sets: batch i + 1 loads while batch i multiplies and
+// stores.  Rows of a batch are read before any of them is written, so D may
+// overwrite A's storage row for row.
+template <int SM, int RT = (SM <= 32 ? 2 : 1), bool PP = (SM <= 64)>
+__device__ __forceinline__ void wv_tall_mm(int NB, int kd, int nc, double alpha, WvSrc A, const double *Kl, int ldk,
+                                           double *D, int t0 = 0, int tstep = 1)
+{
+    const int rstep = 16 * tstep, bstep = rstep * RT;  // rows between tiles, between batches
+    int r0 = 16 * t0;
+    const wv_rsrc dr = wv_col_rsrc(D, NB * nc * 8);
+    const int c0 = A.c0 < kd ? A.c0 : kd;
+    const wv_rsrc r0s = wv_col_rsrc(A.s0, c0 * A.lda * 8), r1s = wv_col_rsrc(A.s1, (kd - c0) * A.lda * 8);
+    if constexpr (PP) {
+        double kv[SM / 4][SM / 16], kx[SM / 16];
+        wv_tall_k<SM>(kv, kx, kd, c0, Kl, ldk);
+        double a0[RT][SM / 4], x0[RT], a1[RT][SM / 4], x1[RT];
+        if (r0 < NB) wv_tall_ld<SM, RT>(a0, x0, NB, kd, r0s, r1s, c0, A.lda, r0, rstep);
+#pragma unroll 1
+        for (; r0 < NB; r0 += 2 * bstep) {
+            if (r0 + bstep < NB) wv_tall_ld<SM, RT>(a1, x1, NB, kd, r0s, r1s, c0, A.lda, r0 + bstep, rstep);
+            wv_tall_mul<SM, RT>(a0, x0, kv, kx, NB, kd, nc, alpha, dr, r0, rstep);
+            if (r0 + bstep >= NB) break;
+            if (r0 + 2 * bstep < NB) wv_tall_ld<SM, RT>(a0, x0, NB, kd, r0s, r1s, c0, A.lda, r0 + 2 * bstep, rstep);
+            wv_tall_mul<SM, RT>(a1, x1, kv, kx, NB, kd, nc, alpha, dr, r0 + bstep, rstep);
+        }
+    } else {
+#pragma unroll 1
+        for (; r0 < NB; r0 += bstep) {
+            double av[RT][SM / 4], ax[RT];
+            wv_tall_ld<SM, RT>(av, ax, NB, kd, r0s, r1s, c0, A.lda, r0, rstep);
+            wv_tall_mul_lds<SM, RT>(av, ax, NB, kd, c0, nc, alpha, Kl, ldk, dr, r0, rstep);
         }
     }
 }
@@ -177,20 +307,8 @@ __device__ __forceinline__ double wv_skip_floor(double f2, int s, double acc, in
     return fmax(nz, tz);
 }
 
-// Column c of the stack: first pass from the tiles, afterwards from W.
-struct WvCols {
-    const double *src0, *src1;
-    int c0;
-    double *W;
-    int NB;
-    __device__ __forceinline__ const double *col(int c, bool first) const
-    {
-        if (!first) return W + (size_t)c * NB;
-        return c < c0 ? src0 + (size_t)c * NB : src1 + (size_t)(c - c0) * NB;
-    }
-};
-
-// Buffer resources of the stack's columns for wv_geqrf_r (the fields as plain
+// Buffer resources of the columns of a stack [src0 (c0 columns) | src1] in
+// its first pass and of the workspace W afterwards (the fields are plain
 // scalars: no indexed access to a structure in private memory).
 struct WvRs {
     const double *src0, *src1, *W;
@@ -487,24 +605,28 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
         //      k runs over the NB rows four at a time ----
         if (!bdense) {
             const int i16 = lane & 15, k4 = lane >> 4;
-            constexpr int TT = SM / 16, U = 4;
+            constexpr int TT = SM / 16, U = SM <= 32 ? 4 : (SM <= 48 ? 2 : 4);
             wv_d4 xacc[TT][TT];
 #pragma unroll
             for (int ta = 0; ta < TT; ta++)
 #pragma unroll
                 for (int tb = 0; tb < TT; tb++) xacc[ta][tb] = wv_d4{0.0, 0.0, 0.0, 0.0};
-            for (int r0 = 0; r0 < NB; r0 += 4 * U) {
-                double av[U][TT], bv[U][TT];
+            // rows r0 .. r0 + 4U - 1 of Av's and Bv's columns by buffer loads;
+            // out-of-range lanes read 0 (no branch, no select)
+            const wv_rsrc ar = wv_col_rsrc(Av, ra * NB * 8), br = wv_col_rsrc(Bv, rb * NB * 8);
+            auto ldx = [&](double (&av)[U][TT], double (&bv)[U][TT], int r0) {
 #pragma unroll
                 for (int u = 0; u < U; u++) {
                     const int row = r0 + 4 * u + k4;
 #pragma unroll
                     for (int tt = 0; tt < TT; tt++) {
                         const int a = 16 * tt + i16;
-                        av[u][tt] = (16 * tt < ra && a < ra && row < NB) ? Av[row + (size_t)a * NB] : 0.0;
-                        bv[u][tt] = (16 * tt < rb && a < rb && row < NB) ? Bv[row + (size_t)a * NB] : 0.0;
+                        av[u][tt] = wv_ld(ar, (a < ra && row < NB) ? (row + a * NB) * 8 : WV_OOB);
+                        bv[u][tt] = wv_ld(br, (a < rb && row < NB) ? (row + a * NB) * 8 : WV_OOB);
                     }
                 }
+            };
+            auto mulx = [&](const double (&av)[U][TT], const double (&bv)[U][TT]) {
 #pragma unroll
                 for (int u = 0; u < U; u++)
 #pragma unroll
@@ -514,6 +636,27 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
                             for (int tb = 0; tb < TT; tb++)
                                 if (16 * tb < rb) xacc[ta][tb] = mfma16(av[u][ta], bv[u][tb], xacc[ta][tb]);
                         }
+            };
+            if constexpr (SM <= 48) {
+                // two register sets in turn: rows of the next step load
+                // while the current ones multiply
+                double av0[U][TT], bv0[U][TT], av1[U][TT], bv1[U][TT];
+                ldx(av0, bv0, 0);
+#pragma unroll 1
+                for (int r0 = 0; r0 < NB; r0 += 8 * U) {
+                    if (r0 + 4 * U < NB) ldx(av1, bv1, r0 + 4 * U);
+                    mulx(av0, bv0);
+                    if (r0 + 4 * U >= NB) break;
+                    if (r0 + 8 * U < NB) ldx(av0, bv0, r0 + 8 * U);
+                    mulx(av1, bv1);
+                }
+            } else {  // one set (the registers of the widest class are taken)
+#pragma unroll 1
+                for (int r0 = 0; r0 < NB; r0 += 4 * U) {
+                    double av[U][TT], bv[U][TT];
+                    ldx(av, bv, r0);
+                    mulx(av, bv);
+                }
             }
             // X^T into LDS as Xl[b * SM + a], zero outside ra x rb
             double *Xl = L;
@@ -533,7 +676,7 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             wv_sync();
         }
         // ---- Qn = -Bu X^T (NB x ra): K = X^T, K[b][a] = Xl[b * SM + a] ----
-        if (!bdense) wv_tall_mm<SM>(NB, rb, ra, -1.0, [&](int c) { return Bu + (size_t)c * NB; }, L, SM, Qn);
+        if (!bdense) wv_tall_mm<SM>(NB, rb, ra, -1.0, WvSrc{Bu, rb, Bu, NB}, L, SM, Qn);
         wv_sync();
         // ---- R factors of P = [Cu Au] and Q = [Cv Qn] ----
         for (int side = 0; side < 2; side++) {
@@ -735,11 +878,7 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             //      Cv = columns rc .. 2rc-1): U' first when nr <= rc, V'
             //      first otherwise. ----
             auto v_out = [&]() {  // Kv in LDS
-                wv_tall_mm<SM>(NB, s, nr, 1.0,
-                               [&](int c) {
-                                   return c < rc ? (const double *)C + (size_t)(rc + c) * NB : Qs + (size_t)(c - rc) * NB;
-                               },
-                               Vl, SM, C + (size_t)nr * NB);
+                wv_tall_mm<SM>(NB, s, nr, 1.0, WvSrc{C + (size_t)rc * NB, rc, Qs, NB}, Vl, SM, C + (size_t)nr * NB);
                 wv_sync();
             };
             auto u_out = [&]() {  // stages Ku into LDS
@@ -748,9 +887,7 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
                     Vl[q] = (j < nr) ? Ku[c * SM + j] : 0.0;
                 }
                 wv_sync();
-                wv_tall_mm<SM>(NB, s, nr, 1.0,
-                               [&](int c) { return c < rc ? (const double *)C + (size_t)c * NB : Au + (size_t)(c - rc) * NB; },
-                               Vl, SM, C);
+                wv_tall_mm<SM>(NB, s, nr, 1.0, WvSrc{C, rc, Au, NB}, Vl, SM, C);
                 wv_sync();
             };
             if (nr > rc) {
@@ -804,10 +941,11 @@ __host__ __device__ inline size_t wb_slot_doubles(int NB)
 // the two then update alternate trailing columns.  Every wave of the
 // workgroup must call it with the same s (the barriers are workgroup-wide).
 // R is written column-major (R(i, j) at Rout[i + j * WB_SM]).
-__device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout, int part, double *sh)
+__device__ __forceinline__ void wb_geqrf_r(int s, const double *src0, const double *src1, int c0, double *W, int NB,
+                                           double *Rout, int part, double *sh)
 {
     const int lane = threadIdx.x & 63;
-    const int NB = A.NB;
+    const WvRs A{src0, src1, W, c0, NB};
     for (int j0 = 0; j0 < s; j0 += WV_PW) {
         const int jb = min(WV_PW, s - j0);
         const bool first = (j0 == 0);
@@ -815,16 +953,18 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout,
         double a[WV_KMAX][WV_PW];
         double T[WV_PW][WV_PW];
         double m0[WV_PW], m1[WV_PW];
+        // this wave's trailing columns t0, t0 + 2, ... in two register sets
+        // used in turn (the next column loads while the current one updates)
+        const int t0 = j0 + jb + part;
+        double p_[WV_KMAX], q_[WV_KMAX];
         if (part == 0) {
 #pragma unroll
             for (int c = 0; c < WV_PW; c++) {
-                const double *col = A.col(j0 + c, first);
+                const wv_rsrc r = A.rd(j0 + c, first, c < jb);
 #pragma unroll
-                for (int k = 0; k < WV_KMAX; k++) {
-                    const int i = lane + 64 * k;
-                    a[k][c] = (c < jb && i < NB) ? col[i] : 0.0;
-                }
+                for (int k = 0; k < WV_KMAX; k++) a[k][c] = wv_ld(r, (lane + 64 * k) * 8);
             }
+            wv_ld_col(p_, A.rd(t0, first, t0 < s), lane);
 #pragma unroll
             for (int r = 0; r < WV_PW; r++)
 #pragma unroll
@@ -929,6 +1069,7 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout,
         }
         __syncthreads();
         if (part == 1) {
+            wv_ld_col(p_, A.rd(t0, first, t0 < s), lane);
 #pragma unroll
             for (int c = 0; c < WV_PW; c++) {
                 m0[c] = sh[(0 * WV_PW + c) * 64 + lane];
@@ -943,29 +1084,7 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout,
         }
         // trailing columns j0 + jb + part, + 2, ...: the two waves of the
         // side take alternate columns
-        double nb_[WV_KMAX];
-        {
-            const int t = j0 + jb + part;
-            const double *col = A.col(t < s ? t : j0, first);
-#pragma unroll
-            for (int k = 0; k < WV_KMAX; k++) {
-                const int i = lane + 64 * k;
-                nb_[k] = (t < s && i < NB) ? col[i] : 0.0;
-            }
-        }
-        for (int t = j0 + jb + part; t < s; t += 2) {
-            double b[WV_KMAX];
-#pragma unroll
-            for (int k = 0; k < WV_KMAX; k++) b[k] = nb_[k];
-            {
-                const bool nx = t + 2 < s;
-                const double *col = A.col(nx ? t + 2 : t, first);
-#pragma unroll
-                for (int k = 0; k < WV_KMAX; k++) {
-                    const int i = lane + 64 * k;
-                    nb_[k] = (nx && i < NB) ? col[i] : 0.0;
-                }
-            }
+        auto upd = [&](double (&b)[WV_KMAX], int t) {
             double p[WV_PW];
 #pragma unroll
             for (int c = 0; c < WV_PW; c++) {
@@ -997,12 +1116,14 @@ __device__ __forceinline__ void wb_geqrf_r(int s, const WvCols &A, double *Rout,
                 for (int c = 0; c < WV_PW; c++) acc = fma(-a[k][c], w[c], acc);
                 b[k] = acc;
             }
-            double *out = A.W + (size_t)t * NB;
-#pragma unroll
-            for (int k = 0; k < WV_KMAX; k++) {
-                const int i = lane + 64 * k;
-                if (i < NB) out[i] = b[k];
-            }
+            wv_st_col(b, A.wr(t, true), lane);
+        };
+        for (int t = t0; t < s; t += 4) {
+            wv_ld_col(q_, A.rd(t + 2, first, t + 2 < s), lane);
+            upd(p_, t);
+            if (t + 2 >= s) break;
+            wv_ld_col(p_, A.rd(t + 4, first, t + 4 < s), lane);
+            upd(q_, t + 2);
         }
         __syncthreads();  // both waves' trailing columns are in W before the next panel
     }
@@ -1096,7 +1217,7 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
         }
         // ---- Qn = -Bu X^T ----
         if (!bdense)
-            wv_tall_mm<WB_SM>(NB, rb, ra, -1.0, [&](int c) { return Bu + (size_t)c * NB; }, L, WB_SM, Qn, wid,
+            wv_tall_mm<WB_SM>(NB, rb, ra, -1.0, WvSrc{Bu, rb, Bu, NB}, L, WB_SM, Qn, wid,
                               WB_WAVES);
         __syncthreads();
         // ---- R factors: P = [Cu Au] on waves 0 and 2 (trailing copy in Wq),
@@ -1105,8 +1226,8 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
         //      free until the Jacobi ----
         {
             const int side = wid & 1;
-            WvCols S{side ? C + (size_t)NB * rc : C, side ? Qs : Au, rc, side ? Out : Wq, NB};
-            wb_geqrf_r(s, S, side ? Rq : Rp, wid >> 1, L + (size_t)side * WB_QR_SH);
+            wb_geqrf_r(s, side ? C + (size_t)NB * rc : C, side ? Qs : Au, rc, side ? Out : Wq, NB, side ? Rq : Rp,
+                       wid >> 1, L + (size_t)side * WB_QR_SH);
         }
         __syncthreads();
         // ---- M^T = Rq Rp^T on the matrix cores (K = Rp^T = Rp's column-major
@@ -1117,7 +1238,7 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             L[q] = (c < s) ? Rp[q] : 0.0;  // K[l][c] = Rp(c, l)
         }
         __syncthreads();
-        wv_tall_mm<WB_SM>(s, s, s, 1.0, [&](int l) { return (const double *)Rq + (size_t)l * WB_SM; }, L, WB_SM, Wq,
+        wv_tall_mm<WB_SM>(s, s, s, 1.0, WvSrc{Rq, s, Rq, WB_SM}, L, WB_SM, Wq,
                           wid, WB_WAVES);
         __syncthreads();
         for (int q = tid; q < s * WB_LD; q += NTH) {
@@ -1309,12 +1430,8 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             // ---- outputs in place into the tile, ordered as in k_rc_wave (each
             //      wave reads and writes only its own row tiles) ----
             auto v_out = [&]() {  // Kv in LDS
-                wv_tall_mm<WB_SM>(NB, s, nr, 1.0,
-                                  [&](int c) {
-                                      return c < rc ? (const double *)C + (size_t)(rc + c) * NB
-                                                    : Qs + (size_t)(c - rc) * NB;
-                                  },
-                                  Vl, WB_SM, C + (size_t)nr * NB, wid, WB_WAVES);
+                wv_tall_mm<WB_SM>(NB, s, nr, 1.0, WvSrc{C + (size_t)rc * NB, rc, Qs, NB}, Vl, WB_SM,
+                                  C + (size_t)nr * NB, wid, WB_WAVES);
                 __syncthreads();
             };
             auto u_out = [&]() {  // stages Ku into LDS
@@ -1323,10 +1440,7 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
                     Vl[q] = (j < nr) ? Ku[c * WB_SM + j] : 0.0;
                 }
                 __syncthreads();
-                wv_tall_mm<WB_SM>(
-                    NB, s, nr, 1.0,
-                    [&](int c) { return c < rc ? (const double *)C + (size_t)c * NB : Au + (size_t)(c - rc) * NB; },
-                    Vl, WB_SM, C, wid, WB_WAVES);
+                wv_tall_mm<WB_SM>(NB, s, nr, 1.0, WvSrc{C, rc, Au, NB}, Vl, WB_SM, C, wid, WB_WAVES);
                 __syncthreads();
             };
             if (nr > rc) {

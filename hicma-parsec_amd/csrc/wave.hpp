@@ -132,6 +132,7 @@ __device__ __forceinline__ void wv_sync()
 // against vmcnt and are issued from a scalar base, so loads issued ahead of
 // use are waited for at their use, not at the next LDS or scalar access.
 typedef unsigned int wv_u2 __attribute__((ext_vector_type(2)));
+#define WV_OOB 0x80000000  // a byte offset past every resource: load 0, store dropped
 typedef __amdgpu_buffer_rsrc_t wv_rsrc;
 __device__ __forceinline__ wv_rsrc wv_col_rsrc(const double *p, int bytes)
 {
