@@ -725,9 +725,11 @@ static int launch_potrf(hicma_tlr_desc *A, int k, double *Winv, double *Linv, in
     TLR &t = A->t;
     const int nb = (t.NB + PBK - 1) / PBK;
     hipLaunchKernelGGL(k_fill_i32, dim3((nb * nb + 2 + 255) / 256), dim3(256), 0, s, dflags, nb * nb + 2, 0);
-    // one workgroup per block column: a whole column of tasks can run at once
-    hipLaunchKernelGGL(k_potrf_dag, dim3(nb), dim3(BS), sizeof(double) * DAG_SMEM, s, t, k, Winv, Linv, dflags,
-                       A->ctx->d_info, A->ctx->d_info + 3, o);
+    // one worker per block column (a whole column of tasks can run at once),
+    // on the XCD that the kernel elects among DAG_XCD_SPREAD times as many
+    // workgroups
+    hipLaunchKernelGGL(k_potrf_dag, dim3(nb * DAG_XCD_SPREAD), dim3(BS), sizeof(double) * DAG_SMEM, s, t, k, Winv,
+                       Linv, dflags, A->ctx->d_info, A->ctx->d_info + 3, o);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -967,27 +969,30 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
                 }
                 hipLaunchKernelGGL(k_classify_wave, dim3((unsigned)((ntask + 255) / 256)), dim3(256), 0, st, t, k,
                                    tasks_s, wlist, wcount, wcap, wide_only, qnws ? 1 : 0);
-                // fork: variants 32 and 48 on the context stream, 64 and the
-                // wide one on side streams 0 and 1
+                // fork: the wide variant and variant 64 on side streams 1 and 0,
+                // variants 48 and 32 on the context stream.  Launched longest
+                // tasks first: the dispatcher hands out CUs in launch order, so
+                // the long tasks start at once and the short ones fill the
+                // step's tail.
                 HIPCHK(hipEventRecord(ctx->ev_fork, st));
                 for (int i = 0; i < 2; i++) HIPCHK(hipStreamWaitEvent(ctx->side[i], ctx->ev_fork, 0));
-                int wg = (int)std::min<long long>((wgrid - wres) * WvCfg<32>::BPC, (ntask + WvCfg<32>::WPB - 1) / WvCfg<32>::WPB);
-                hipLaunchKernelGGL(k_rc_wave<32>, dim3(wg), dim3(64 * WvCfg<32>::WPB), wlds32, st, t, k,
-                                   (const int *)wlist, (const int *)wcount, wcount + 4, tasks_s, wws, acc, fixed_rank,
-                                   maxrank, o, ctx->d_ovf, (const double *)qnws);
-                wg = (int)std::min<long long>((wgrid - wres) * WvCfg<48>::BPC, (ntask + WvCfg<48>::WPB - 1) / WvCfg<48>::WPB);
-                hipLaunchKernelGGL(k_rc_wave<48>, dim3(wg), dim3(64 * WvCfg<48>::WPB), wlds48, st, t, k,
-                                   (const int *)(wlist + wcap), (const int *)(wcount + 1), wcount + 5, tasks_s,
-                                   wws + wws_d / 4, acc, fixed_rank, maxrank, o, ctx->d_ovf, (const double *)qnws);
+                int wg = (int)std::min<long long>(wgrid - wres, ntask);
+                hipLaunchKernelGGL(k_rc_wide, dim3(wg), dim3(64 * WB_WAVES), sizeof(double) * WB_LDS, ctx->side[1], t, k,
+                                   (const int *)(wlist + 3 * (size_t)wcap), (const int *)(wcount + 3), wcount + 7,
+                                   tasks_s, wbs, acc, fixed_rank, maxrank, o1, ctx->d_ovf, (const double *)qnws);
                 wg = (int)std::min<long long>((wgrid - wres) * WvCfg<64>::BPC, (ntask + WvCfg<64>::WPB - 1) / WvCfg<64>::WPB);
                 hipLaunchKernelGGL(k_rc_wave<64>, dim3(wg), dim3(64 * WvCfg<64>::WPB), wlds64, ctx->side[0], t, k,
                                    (const int *)(wlist + 2 * (size_t)wcap), (const int *)(wcount + 2), wcount + 6,
                                    tasks_s, wws + wws_d / 2, acc, fixed_rank, maxrank, o0, ctx->d_ovf,
                                    (const double *)qnws);
-                wg = (int)std::min<long long>(wgrid - wres, ntask);
-                hipLaunchKernelGGL(k_rc_wide, dim3(wg), dim3(64 * WB_WAVES), sizeof(double) * WB_LDS, ctx->side[1], t, k,
-                                   (const int *)(wlist + 3 * (size_t)wcap), (const int *)(wcount + 3), wcount + 7,
-                                   tasks_s, wbs, acc, fixed_rank, maxrank, o1, ctx->d_ovf, (const double *)qnws);
+                wg = (int)std::min<long long>((wgrid - wres) * WvCfg<48>::BPC, (ntask + WvCfg<48>::WPB - 1) / WvCfg<48>::WPB);
+                hipLaunchKernelGGL(k_rc_wave<48>, dim3(wg), dim3(64 * WvCfg<48>::WPB), wlds48, st, t, k,
+                                   (const int *)(wlist + wcap), (const int *)(wcount + 1), wcount + 5, tasks_s,
+                                   wws + wws_d / 4, acc, fixed_rank, maxrank, o, ctx->d_ovf, (const double *)qnws);
+                wg = (int)std::min<long long>((wgrid - wres) * WvCfg<32>::BPC, (ntask + WvCfg<32>::WPB - 1) / WvCfg<32>::WPB);
+                hipLaunchKernelGGL(k_rc_wave<32>, dim3(wg), dim3(64 * WvCfg<32>::WPB), wlds32, st, t, k,
+                                   (const int *)wlist, (const int *)wcount, wcount + 4, tasks_s, wws, acc, fixed_rank,
+                                   maxrank, o, ctx->d_ovf, (const double *)qnws);
                 // join
                 for (int i = 0; i < 2; i++) {
                     HIPCHK(hipEventRecord(ctx->ev_join[i], ctx->side[i]));

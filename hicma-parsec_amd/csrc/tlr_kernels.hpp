@@ -159,57 +159,117 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Factors the jb x jb block held (lower part, row major) in Lr[i * PLR + c]
-// with ONE wave and forms W = L^-1: right-looking Cholesky with lane i owning
-// row i in registers (pivots by readlane, each column of L broadcast through
-// LDS, so each step issues up to 63 independent FMAs), then W column by
-// column with lane c owning column c.  On return Lr holds L (zero above the
-// diagonal and outside jb) and Wc[c * PLR + i] = W(i, c).  Returns the
-// 1-based index of the first non-positive pivot, or 0.
-__device__ int diag_factor_wave(double *Lr, double *Wc, int jb)
+// ---- 64 x 64 Cholesky and inverse by a workgroup of four waves, blocked by
+// 16: per diagonal block b, one wave factors the 16 x 16 block and inverts
+// it (16 dependent steps each, pivots and columns broadcast by readlane),
+// then the panel L(below, b) = S(below, b) W_b^T and the trailing update
+// S -= L L^T run as 16 x 16 MFMA tiles over the four waves; afterwards the
+// off-diagonal blocks of W = L^-1, W_IJ = -W_II sum_{l=J}^{I-1} L_Il W_lJ,
+// in three rounds of MFMA tiles.
+// Lr: the matrix (row-major, ld PLR; lower part, rows of the padding hold
+// the identity); on return its lower part is L.  Wc: W(r, c) at
+// Wc[c * PLR + r].  Ts: 4 x 16 x 17 doubles of scratch.  Called by all 256
+// threads; returns the 1-based index of the first non-positive pivot, or 0.
+__device__ __forceinline__ double d16_rd(double v, int l)
 {
-    const int lane = threadIdx.x & 63;
-    double a[PBK];
+    return mk64((unsigned)__builtin_amdgcn_readlane((int)lo32(v), l),
+                (unsigned)__builtin_amdgcn_readlane((int)hi32(v), l));
+}
+
+__device__ int diag_factor_blk(double *Lr, double *Wc, double *Ts, int *s_bad)
+{
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, i16 = lane & 15, k4 = lane >> 4;
+    if (tid == 0) *s_bad = 0;
+    __syncthreads();
+    for (int b = 0; b < PBK; b += 16) {
+        if (w == 0) {
+            // factor: lane i (mod 16) holds row i of the block
+            const int i = i16;
+            double a[16];
 #pragma unroll
-    for (int c = 0; c < PBK; c++) a[c] = Lr[lane * PLR + c];
-    double *colj = Wc;
+            for (int c = 0; c < 16; c++) a[c] = Lr[(b + i) * PLR + b + c];
+            int badj = 0;
 #pragma unroll
-    for (int j = 0; j < PBK; j++) {
-        if (j < jb) {
-            const double d2 = mk64((unsigned)__builtin_amdgcn_readlane((int)lo32(a[j]), j),
-                                   (unsigned)__builtin_amdgcn_readlane((int)hi32(a[j]), j));
-            if (!(d2 > 0.0)) return j + 1;
-            const double d = sqrt(d2);
-            const double lij = (lane == j) ? d : ((lane > j && lane < jb) ? a[j] / d : 0.0);
-            a[j] = lij;
-            colj[lane] = lij;
-            wave_lds_sync();
+            for (int j = 0; j < 16; j++) {
+                const double d2 = d16_rd(a[j], j);
+                if (!(d2 > 0.0) && badj == 0) badj = j + 1;
+                const double d = sqrt(d2);
+                const double lij = (i == j) ? d : (i > j ? a[j] / d : 0.0);
+                a[j] = lij;
 #pragma unroll
-            for (int c = j + 1; c < PBK; c++) a[c] = fma(-lij, colj[c], a[c]);
-            wave_lds_sync();
+                for (int c = j + 1; c < 16; c++) a[c] = fma(-lij, d16_rd(lij, c), a[c]);
+            }
+            // inverse: lane c (mod 16) holds column c of W
+            double x[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) x[r] = (r == i) ? 1.0 : 0.0;
+#pragma unroll
+            for (int l = 0; l < 16; l++) {
+                const double xl = x[l] / d16_rd(a[l], l);
+                x[l] = xl;
+#pragma unroll
+                for (int r = l + 1; r < 16; r++) x[r] = fma(-d16_rd(a[l], r), xl, x[r]);
+            }
+            if (lane < 16) {
+#pragma unroll
+                for (int c = 0; c < 16; c++) Lr[(b + i) * PLR + b + c] = (c <= i) ? a[c] : 0.0;
+#pragma unroll
+                for (int r = 0; r < 16; r++) Wc[(b + i) * PLR + b + r] = x[r];
+            }
+            if (lane == 0 && badj && *s_bad == 0) *s_bad = b + badj;
         }
-    }
-    // L (lower part of the rows) back to LDS
+        __syncthreads();
+        const int nt = (PBK - b - 16) / 16;  // 16-row tiles below the block
+        // panel: L(rt + i, b + j) = sum_k S(rt + i, b + k) W(b + j, b + k)
+        if (w < nt) {
+            const int rt = b + 16 + 16 * w;
+            mk_d4 acc = mk_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int c = 0; c < PBK; c++) Lr[lane * PLR + c] = (c <= lane && lane < jb) ? a[c] : 0.0;
-    wave_lds_sync();
-    // W = L^-1, right-looking, lane = column c: w(l) = x(l) / L(l, l), then
-    // x(i) -= L(i, l) w(l) for i > l (L entries are broadcast LDS reads)
+            for (int k0 = 0; k0 < 16; k0 += 4)
+                acc = mk_mfma(Lr[(rt + i16) * PLR + b + k0 + k4], Wc[(b + k0 + k4) * PLR + b + i16], acc);
 #pragma unroll
-    for (int i = 0; i < PBK; i++) a[i] = (i == lane) ? 1.0 : 0.0;
-#pragma unroll
-    for (int l = 0; l < PBK; l++) {
-        if (l < jb) {
-            const double w = (lane <= l) ? a[l] / Lr[l * PLR + l] : 0.0;
-            a[l] = w;
-#pragma unroll
-            for (int i = l + 1; i < PBK; i++) a[i] = fma(-Lr[i * PLR + l], w, a[i]);
+            for (int r = 0; r < 4; r++) Lr[(rt + k4 + 4 * r) * PLR + b + i16] = acc[r];
         }
-    }
+        __syncthreads();
+        // trailing update of the lower block tiles (ti, tj), tj <= ti
+        for (int q = w; q < nt * (nt + 1) / 2; q += 4) {
+            int ti = 0;
+            while ((ti + 1) * (ti + 2) / 2 <= q) ti++;
+            const int tj = q - ti * (ti + 1) / 2;
+            const int R = b + 16 + 16 * ti, Cc = b + 16 + 16 * tj;
+            mk_d4 acc = mk_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int i = 0; i < PBK; i++) Wc[lane * PLR + i] = (i < jb && lane < jb) ? a[i] : 0.0;
-    wave_lds_sync();
-    return 0;
+            for (int k0 = 0; k0 < 16; k0 += 4)
+                acc = mk_mfma(Lr[(R + i16) * PLR + b + k0 + k4], Lr[(Cc + i16) * PLR + b + k0 + k4], acc);
+#pragma unroll
+            for (int r = 0; r < 4; r++) Lr[(R + k4 + 4 * r) * PLR + Cc + i16] -= acc[r];
+        }
+        __syncthreads();
+    }
+    // off-diagonal blocks of W, block row I = 1, 2, 3 (wave J takes block (I, J))
+    for (int I = 1; I < PBK / 16; I++) {
+        if (w < I) {
+            const int J = w;
+            double *T = Ts + w * 16 * 17;
+            mk_d4 acc = mk_d4{0.0, 0.0, 0.0, 0.0};
+            for (int l = J; l < I; l++)
+#pragma unroll
+                for (int k0 = 0; k0 < 16; k0 += 4)
+                    acc = mk_mfma(Lr[(16 * I + i16) * PLR + 16 * l + k0 + k4],
+                                  Wc[(16 * J + i16) * PLR + 16 * l + k0 + k4], acc);
+#pragma unroll
+            for (int r = 0; r < 4; r++) T[(k4 + 4 * r) * 17 + i16] = acc[r];  // T(i, j)
+            wave_lds_sync();
+            mk_d4 acc2 = mk_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k0 = 0; k0 < 16; k0 += 4)
+                acc2 = mk_mfma(Wc[(16 * I + k0 + k4) * PLR + 16 * I + i16], T[(k0 + k4) * 17 + i16], acc2);
+#pragma unroll
+            for (int r = 0; r < 4; r++) Wc[(16 * J + i16) * PLR + 16 * I + k4 + 4 * r] = -acc2[r];
+        }
+        __syncthreads();
+    }
+    return *s_bad;
 }
 
 // ---- the diagonal tile's Cholesky and its inverse as one launch: a
@@ -229,14 +289,23 @@ __device__ int diag_factor_wave(double *Lr, double *Wc, int jb)
 // deadlock; each wait is bounded all the same (a wait that runs out marks
 // *err and the task goes on, so the kernel always ends).
 //
-// flags: [0] task counter, [1] unused, [2 + task] done; zero at launch.  A
-// wait that runs out sets *err (sticky over the factorization; the host
-// reports it as an error).
-// Publishing a block: its stores, an agent-scope release fence in every
-// thread (stores complete, L2 written back: another XCD may read the block),
-// a barrier and the done flag; reading: the flags, one agent-scope acquire
-// fence (L1 and L2 invalidated) for all dependencies already done, a barrier.
+// flags: [0] task counter, [1] elected XCD + 1, [2 + task] done; zero at
+// launch.  A wait that runs out sets *err (sticky over the factorization; the
+// host reports it as an error).
+//
+// All workers run on ONE XCD, so blocks pass between them through that XCD's
+// L2, which all its CUs share: the launch has DAG_XCD_SPREAD workgroups per
+// worker, the first workgroup to start elects its XCD (hardware register
+// XCC_ID) and workgroups on other XCDs leave at once.  Publishing a block:
+// its stores, a wait in every thread until they have completed (the vector L1
+// writes through to L2), a barrier and the done flag; reading: the flags,
+// one invalidation of this CU's L1 for all dependencies already done, a
+// barrier.  No L2 write-back or invalidation is needed (an agent-scope fence
+// would write back or drop the whole L2, which the concurrent kernels use).
+// The kernel's end publishes the results to other XCDs as every kernel end
+// does.
 #define DAG_SPIN_LIMIT (1 << 21)
+#define DAG_XCD_SPREAD 8
 #define DAG_SMEM (2 * PBK * PLR + MK_SMEM)
 
 __device__ __forceinline__ int dag_fac(int i, int j, int nb) { return j * nb + (i - j); }
@@ -267,7 +336,8 @@ __device__ __forceinline__ int dag_acquire(int *flags, int l, int lend, Dep dep,
         }
         int r = l + 1;
         while (r < lend && done(r)) r++;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // drop this CU's L1 copies: the blocks are read from the XCD's L2
+        asm volatile("buffer_inv sc0\n\ts_waitcnt vmcnt(0)" ::: "memory");
         s_ready = r;
     }
     __syncthreads();
@@ -278,9 +348,9 @@ __device__ __forceinline__ int dag_acquire(int *flags, int l, int lend, Dep dep,
 
 __device__ __forceinline__ void dag_done(int *flags, int task)
 {
-    // every thread's stores of the block complete and are written back from
-    // L2 before the barrier; then the flag
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // every thread's stores of the block have reached the XCD's L2 before
+    // the barrier; then the flag
+    asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0) __hip_atomic_store(flags + 2 + task, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -292,9 +362,20 @@ __global__ void __launch_bounds__(BS) k_potrf_dag(TLR t, int k, double *Winv, do
 {
     extern __shared__ double sm[];
     double *Lr = sm, *Wc = sm + PBK * PLR, *Mk = sm + 2 * PBK * PLR;
-    __shared__ int s_task;
+    __shared__ int s_task, s_bad;
     const int NB = t.NB, nb = (NB + PBK - 1) / PBK, ntask = nb * nb, tid = threadIdx.x;
     double *A = band_tile(t, k, k);
+    {
+        unsigned xcc;
+        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+        const int mine = (int)(xcc & 15u) + 1;
+        if (tid == 0) {
+            const int e = atomicCAS(flags + 1, 0, mine);
+            s_task = (e == 0) ? mine : e;
+        }
+        __syncthreads();
+        if (s_task != mine) return;  // another XCD runs the schedule
+    }
     for (;;) {
         __syncthreads();  // s_task and the LDS buffers are free
         if (tid == 0) s_task = atomicAdd(flags, 1);
@@ -353,23 +434,25 @@ __global__ void __launch_bounds__(BS) k_potrf_dag(TLR t, int k, double *Winv, do
                 mk_accum(op, ri, cj, 0, 0, (long)PBK * l, (long)PBK * l + PBK, acc, Mk);
             }
             if (i == j) {
-                // S (lower) row major into Lr, factor and invert with wave 0
+                // S (lower) row major into Lr, the identity on the padding;
+                // factor and invert with the four waves
                 mk_store(acc, [&](int ii, int jj, double v) {
-                    Lr[ii * PLR + jj] = (ii < cj && jj <= ii) ? A[r0 + ii + (size_t)(c0 + jj) * NB] - v : 0.0;
+                    Lr[ii * PLR + jj] = (ii < cj && jj <= ii) ? A[r0 + ii + (size_t)(c0 + jj) * NB] - v
+                                                              : (ii == jj ? 1.0 : 0.0);
                 });
                 __syncthreads();
-                if (tid < 64) {
-                    const int bad = diag_factor_wave(Lr, Wc, cj);
-                    if (bad) {
-                        if (tid == 0) atomicCAS(info, 0, k * NB + c0 + bad);
-                    } else {
-                        // the whole 64 x 64 W_j, zero outside cj x cj (the trsm
-                        // multiplies the padding by zero rows)
-                        double *Wo = Winv + (size_t)j * PBK * PBK;
-                        for (int c = 0; c < PBK; c++) {
-                            if (c < cj && tid < cj && tid >= c) A[r0 + tid + (size_t)(c0 + c) * NB] = Lr[tid * PLR + c];
-                            Wo[tid + c * PBK] = Wc[c * PLR + tid];
-                        }
+                const int bad = diag_factor_blk(Lr, Wc, Mk, &s_bad);
+                if (bad) {
+                    if (tid == 0) atomicCAS(info, 0, k * NB + c0 + bad);
+                } else {
+                    // the whole 64 x 64 W_j, zero above the diagonal and outside
+                    // cj x cj (the trsm multiplies the padding by zero rows)
+                    double *Wo = Winv + (size_t)j * PBK * PBK;
+                    for (int e = tid; e < PBK * PBK; e += BS) {
+                        const int ii = e % PBK, c = e / PBK;
+                        const bool in = c < cj && ii < cj && ii >= c;
+                        if (in) A[r0 + ii + (size_t)(c0 + c) * NB] = Lr[ii * PLR + c];
+                        Wo[ii + c * PBK] = in ? Wc[c * PLR + ii] : 0.0;
                     }
                 }
             } else {
