@@ -705,10 +705,12 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             }
             wv_sync();
         }
-        // ---- one-sided Jacobi on the columns of M^T (lane c owns column c) ----
+        // ---- one-sided Jacobi on the columns of M^T (lane c owns column c);
+        //      a lane reads its partner's column with ds_bpermute (no LDS
+        //      storage, no wave barrier); columns beyond s are zero, so every
+        //      loop runs over all SM entries without guards ----
         double fro = 0.0;  // ||M||_F
         {
-            double *J = L;
             const int npad = s + (s & 1);
             const double tol = 4.0e-16 * sqrt((double)s);
             double al = 0.0;
@@ -730,27 +732,18 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
                 int rot_any = 0;
                 int pos = p0;  // column at position p in round r: ((p - 1 + r) mod (npad - 1)) + 1
                 for (int r = 0; r < npad - 1; r++) {
-                    // columns beyond s are zero, so every loop runs over all SM
-                    // entries without guards
-                    if (lane < s) {
-                        double *my = J + lane * Cf::LDJ;
-#pragma unroll
-                        for (int q = 0; q < SM; q++) my[q] = xm[q];
-                        my[SM] = al;
-                    }
-                    wv_sync();
                     const int pp = npad - 1 - pos;
                     int pc = 0;
                     if (pp != 0) {
                         pc = pp + r;
                         if (pc > npad - 1) pc -= npad - 1;
                     }
-                    if (lane < s && pc < s) {
-                        const double *yp = J + pc * Cf::LDJ;
-                        double yv[SM];
+                    const int src = 4 * pc;  // pc < npad <= 64
+                    double yv[SM];
 #pragma unroll
-                        for (int q = 0; q < SM; q++) yv[q] = yp[q];
-                        const double be = yp[SM];
+                    for (int q = 0; q < SM; q++) yv[q] = wv_bperm(src, xm[q]);
+                    const double be = wv_bperm(src, al);
+                    if (lane < s && pc < s) {
                         double g4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                         for (int q = 0; q < SM; q++) g4[q & 3] = fma(xm[q], yv[q], g4[q & 3]);
@@ -769,7 +762,6 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
                     }
                     // next round: every column but 0 moves one position down
                     if (lane > 0) pos = (pos == 1) ? npad - 1 : pos - 1;
-                    wv_sync();
                 }
                 // exact norms once per sweep
                 al = 0.0;

@@ -125,6 +125,14 @@ __device__ __forceinline__ void wv_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// value of v on lane addr / 4 (ds_bpermute: through the LDS crossbar, no
+// LDS storage)
+__device__ __forceinline__ double wv_bperm(int addr, double v)
+{
+    return mk64((unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)lo32(v)),
+                (unsigned)__builtin_amdgcn_ds_bpermute(addr, (int)hi32(v)));
+}
+
 // Raw buffer access to one column of a tile or workspace.  The resource
 // (gfx9 word 3 = 0x00020000: untyped, unswizzled) bounds the column: a load
 // at or past `bytes` returns 0 and a store there is dropped, so rows past the
