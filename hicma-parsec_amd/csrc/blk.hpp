@@ -48,7 +48,7 @@ __device__ __forceinline__ double gsum_rt(double v, int g)
 // Returns false when no rotation is needed or the values underflow.
 __device__ __forceinline__ bool jacobi_rot(double al, double be, double ga, double tol, double &c, double &s)
 {
-    if (ga == 0.0 || !(fabs(ga) > tol * sqrt(al * be))) return false;
+    if (ga == 0.0 || !(ga * ga > (tol * tol) * (al * be))) return false;  // |ga| > tol sqrt(al be)
     const double diff = be - al;
     const double den = fabs(diff) + sqrt(diff * diff + 4.0 * ga * ga);
     if (!(den > 0.0)) return false;

@@ -918,7 +918,7 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
 
 #define WB_SM 128                       // largest stack width of the wide variant
 #define WB_LD (WB_SM + 1)               // LDS column stride of the Jacobi matrix
-#define WB_LDS (WB_SM * WB_LD + 2 * WB_SM)  // doubles of LDS (one wave per workgroup)
+#define WB_LDS (WB_SM * WB_LD + 3 * WB_SM)  // doubles of LDS (one workgroup per CU)
 #define WB_QR_SH (64 * (WV_KMAX * WV_PW + 1))  // doubles of LDS per QR side (two sides fit in WB_LDS)
 
 __host__ __device__ inline size_t wb_slot_doubles(int NB)
@@ -1238,12 +1238,19 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             J[q] = (r < s) ? Wq[r + (size_t)c * s] : 0.0;
         }
         __syncthreads();
+        // squared column norms, kept up to date through the rotations and
+        // recomputed exactly after every sweep
+        double *nrm = sg + 2 * WB_SM;
+        for (int c = tid; c < s; c += NTH) {
+            const double *x = J + c * WB_LD;
+            double v = 0.0;
+            for (int q = 0; q < s; q++) v = fma(x[q], x[q], v);
+            nrm[c] = v;
+        }
+        __syncthreads();
         if (wid == 0) {  // skip floor of the Jacobi rotations
             double f2 = 0.0;
-            for (int c = lane; c < s; c += 64) {
-                const double *x = J + c * WB_LD;
-                for (int q = 0; q < s; q++) f2 = fma(x[q], x[q], f2);
-            }
+            for (int c = lane; c < s; c += 64) f2 += nrm[c];
             const double f2t = wsum(f2);
             const double nzv = wv_skip_floor(f2t, s, acc, fixed_rk);
             if (lane == 0) {
@@ -1253,8 +1260,8 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
         }
         __syncthreads();
         // ---- one-sided Jacobi, columns in LDS: column pair tid / 4, the four
-        //      lanes of a quad take every fourth entry (quad sums by DPP give
-        //      all four the same al, be, ga, so they agree on the rotation) ----
+        //      lanes of a quad take every fourth entry (the quad sum by DPP
+        //      gives all four the same ga, so they agree on the rotation) ----
         {
             const int npad = s + (s & 1), npairs = npad >> 1;
             const double tol = 4.0e-16 * sqrt((double)s);
@@ -1270,7 +1277,8 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
                         const int cb = (pb == 0) ? 0 : ((pb - 1 + r) % (npad - 1)) + 1;
                         if (ca < s && cb < s) {
                             double *x = J + ca * WB_LD, *y = J + cb * WB_LD;
-                            double a4[4] = {0, 0, 0, 0}, b4[4] = {0, 0, 0, 0}, g4[4] = {0, 0, 0, 0};
+                            const double al = nrm[ca], be = nrm[cb];
+                            double g4[4] = {0, 0, 0, 0};
                             int q = sub;
                             for (; q + 12 < s; q += 16) {
                                 double xv[4], yv[4];
@@ -1280,24 +1288,17 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
                                     yv[u] = y[q + 4 * u];
                                 }
 #pragma unroll
-                                for (int u = 0; u < 4; u++) {
-                                    a4[u] = fma(xv[u], xv[u], a4[u]);
-                                    b4[u] = fma(yv[u], yv[u], b4[u]);
-                                    g4[u] = fma(xv[u], yv[u], g4[u]);
-                                }
+                                for (int u = 0; u < 4; u++) g4[u] = fma(xv[u], yv[u], g4[u]);
                             }
-                            for (; q < s; q += 4) {
-                                const double xv = x[q], yv = y[q];
-                                a4[0] = fma(xv, xv, a4[0]);
-                                b4[0] = fma(yv, yv, b4[0]);
-                                g4[0] = fma(xv, yv, g4[0]);
-                            }
-                            const double al = gsum<4>((a4[0] + a4[1]) + (a4[2] + a4[3]));
-                            const double be = gsum<4>((b4[0] + b4[1]) + (b4[2] + b4[3]));
+                            for (; q < s; q += 4) g4[0] = fma(x[q], y[q], g4[0]);
                             const double ga = gsum<4>((g4[0] + g4[1]) + (g4[2] + g4[3]));
                             double cs, sn;
                             if (fmin(al, be) > nz && jacobi_rot(al, be, ga, tol, cs, sn)) {
                                 rot_any = 1;
+                                if (sub == 0) {  // norms after the rotation
+                                    nrm[ca] = cs * cs * al - 2.0 * cs * sn * ga + sn * sn * be;
+                                    nrm[cb] = sn * sn * al + 2.0 * cs * sn * ga + cs * cs * be;
+                                }
                                 int q2 = sub;
                                 for (; q2 + 12 < s; q2 += 16) {
                                     double xv[4], yv[4];
@@ -1321,6 +1322,12 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
                         }
                     }
                     __syncthreads();
+                }
+                for (int c = tid; c < s; c += NTH) {
+                    const double *x = J + c * WB_LD;
+                    double v = 0.0;
+                    for (int q2 = 0; q2 < s; q2++) v = fma(x[q2], x[q2], v);
+                    nrm[c] = v;
                 }
                 if (!__syncthreads_or(rot_any)) break;
             }
