@@ -693,14 +693,25 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             wv_sync();
 #pragma unroll
             for (int q = 0; q < SM; q++) xm[q] = 0.0;
-            for (int l = 0; l < s; l++) {
-                const double rp = (lane < s && l >= lane) ? Rp[lane * SM + l] : 0.0;
-                const double *rq = L + l * SM;
+            // this lane's row of Rp, 16 entries per batch of loads
+            const wv_rsrc rpr = wv_col_rsrc(Rp, SM * SM * 8);
+            for (int l0 = 0; l0 < s; l0 += 16) {
+                double rv[16];
 #pragma unroll
-                for (int q8 = 0; q8 < SM; q8 += 8)
-                    if (q8 < s) {
+                for (int u = 0; u < 16; u++) {
+                    const int l = l0 + u;
+                    rv[u] = wv_ld(rpr, (lane < s && l < s && l >= lane) ? (lane * SM + l) * 8 : WV_OOB);
+                }
 #pragma unroll
-                        for (int q = q8; q < q8 + 8; q++) xm[q] = fma(rp, rq[q], xm[q]);
+                for (int u = 0; u < 16; u++)
+                    if (l0 + u < s) {
+                        const double *rq = L + (l0 + u) * SM;
+#pragma unroll
+                        for (int q8 = 0; q8 < SM; q8 += 8)
+                            if (q8 < s) {
+#pragma unroll
+                                for (int q = q8; q < q8 + 8; q++) xm[q] = fma(rv[u], rq[q], xm[q]);
+                            }
                     }
             }
             wv_sync();
@@ -809,14 +820,25 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
                 double ac[SM];
 #pragma unroll
                 for (int j = 0; j < SM; j++) ac[j] = 0.0;
-                for (int l = 0; l < s; l++) {
-                    const double rq = (lane < s && l <= lane) ? Rq[l * SM + lane] : 0.0;
-                    const double *vr = Vl + l * SM;
+                // column `lane` of Rq, 16 entries per batch of loads
+                const wv_rsrc rqr = wv_col_rsrc(Rq, SM * SM * 8);
+                for (int l0 = 0; l0 < s; l0 += 16) {
+                    double rv[16];
 #pragma unroll
-                    for (int j8 = 0; j8 < SM; j8 += 8)
-                        if (j8 < nr) {
+                    for (int u = 0; u < 16; u++) {
+                        const int l = l0 + u;
+                        rv[u] = wv_ld(rqr, (lane < s && l < s && l <= lane) ? (l * SM + lane) * 8 : WV_OOB);
+                    }
 #pragma unroll
-                            for (int j = j8; j < j8 + 8; j++) ac[j] = fma(rq, vr[j], ac[j]);
+                    for (int u = 0; u < 16; u++)
+                        if (l0 + u < s) {
+                            const double *vr = Vl + (l0 + u) * SM;
+#pragma unroll
+                            for (int j8 = 0; j8 < SM; j8 += 8)
+                                if (j8 < nr) {
+#pragma unroll
+                                    for (int j = j8; j < j8 + 8; j++) ac[j] = fma(rv[u], vr[j], ac[j]);
+                                }
                         }
                 }
                 if (lane < s) {
