@@ -857,24 +857,30 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             // sigma_j s eps max_q sum_l |Rq(q,l)| |Kv(l,j)|; a task over the
             // tolerance, or with an exactly zero pivot, is left untouched and
             // handed to the Householder path (k_lr_gemm).
+            // Row q of Rq is loaded across the lanes (lane l: Rq(q, l), one
+            // load per row, the next row's issued ahead) and read by readlane.
             bool bad = false;
-            if (lane < nr) {
+            {
+                const wv_rsrc rqr = wv_col_rsrc(Rq, SM * SM * 8);
                 double gmax = 0.0;
+                double rnext = wv_ld(rqr, (lane < s) ? ((s - 1) * SM + lane) * 8 : WV_OOB);
                 for (int q = s - 1; q >= 0; q--) {
+                    const double rrow = rnext;
+                    rnext = wv_ld(rqr, (q > 0 && lane < s) ? ((q - 1) * SM + lane) * 8 : WV_OOB);
                     double v = Vl[q * SM + lane], gq = 0.0;
                     for (int l = q + 1; l < s; l++) {
-                        const double rql = Rq[q * SM + l], xl = Vl[l * SM + lane];
+                        const double rql = rdlane(rrow, l), xl = Vl[l * SM + lane];
                         v = fma(-rql, xl, v);
                         gq = fma(fabs(rql), fabs(xl), gq);
                     }
-                    const double d = Rq[q * SM + q];
+                    const double d = rdlane(rrow, q);
                     const bool ok = fabs(d) > 1e-300;
                     const double x = ok ? v / d : 0.0;
                     bad = bad || !ok;
                     gmax = fmax(gmax, fma(fabs(d), fabs(x), gq));
-                    Vl[q * SM + lane] = x;
+                    if (lane < nr) Vl[q * SM + lane] = x;
                 }
-                bad = bad || !(sgp[lane] * s * 2.3e-16 * gmax <= rc_solve_tol(acc, fixed_rk, fro));
+                bad = lane < nr && (bad || !(sgp[lane] * s * 2.3e-16 * gmax <= rc_solve_tol(acc, fixed_rk, fro)));
             }
             if (__ballot(bad) != 0) {
                 if (lane == 0) tasks_s[e] = RC_REDO;
