@@ -1427,27 +1427,37 @@ k_rc_wide(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             __syncthreads();
             // Kv = Rq^-1 Vs_r in place, thread j solves column j: plain back
             // substitution with the error bound and hand-off of k_rc_wave
+            // waves 0 and 1 (thread j = column j, nr <= 128); row q of Rq is
+            // loaded across the lanes (lane l: Rq(q, l) and Rq(q, l + 64), the
+            // next row's issued ahead) and read by readlane
             int bad = 0;
-            {
+            if (wid < 2) {
                 const double tol = rc_solve_tol(acc, fixed_rk, s_fro);
-                for (int j = tid; j < nr; j += NTH) {
-                    double gmax = 0.0;
-                    for (int q = s - 1; q >= 0; q--) {
-                        double v = Vl[q * WB_SM + j], gq = 0.0;
-                        for (int l = q + 1; l < s; l++) {
-                            const double rql = Rq[q + (size_t)l * WB_SM], xl = Vl[l * WB_SM + j];
-                            v = fma(-rql, xl, v);
-                            gq = fma(fabs(rql), fabs(xl), gq);
-                        }
-                        const double d = Rq[q + (size_t)q * WB_SM];
-                        const bool ok = fabs(d) > 1e-300;
-                        const double x = ok ? v / d : 0.0;
-                        if (!ok) bad = 1;
-                        gmax = fmax(gmax, fma(fabs(d), fabs(x), gq));
-                        Vl[q * WB_SM + j] = x;
+                const int j = tid;
+                const wv_rsrc rqr = wv_col_rsrc(Rq, WB_SM * WB_SM * 8);
+                auto ldrow = [&](int q, double &r0, double &r1) {
+                    r0 = wv_ld(rqr, (q >= 0 && lane < s) ? (q + lane * WB_SM) * 8 : WV_OOB);
+                    r1 = wv_ld(rqr, (q >= 0 && lane + 64 < s) ? (q + (lane + 64) * WB_SM) * 8 : WV_OOB);
+                };
+                double n0, n1, gmax = 0.0;
+                ldrow(s - 1, n0, n1);
+                for (int q = s - 1; q >= 0; q--) {
+                    const double r0 = n0, r1 = n1;
+                    ldrow(q - 1, n0, n1);
+                    double v = Vl[q * WB_SM + j], gq = 0.0;
+                    for (int l = q + 1; l < s; l++) {
+                        const double rql = (l < 64) ? rdlane(r0, l) : rdlane(r1, l - 64), xl = Vl[l * WB_SM + j];
+                        v = fma(-rql, xl, v);
+                        gq = fma(fabs(rql), fabs(xl), gq);
                     }
-                    if (!(sg[WB_SM + j] * s * 2.3e-16 * gmax <= tol)) bad = 1;
+                    const double d = (q < 64) ? rdlane(r0, q) : rdlane(r1, q - 64);
+                    const bool ok = fabs(d) > 1e-300;
+                    const double x = ok ? v / d : 0.0;
+                    if (!ok && j < nr) bad = 1;
+                    gmax = fmax(gmax, fma(fabs(d), fabs(x), gq));
+                    if (j < nr) Vl[q * WB_SM + j] = x;
                 }
+                if (j < nr && !(sg[WB_SM + j] * s * 2.3e-16 * gmax <= tol)) bad = 1;
             }
             if (__syncthreads_or(bad)) {
                 if (tid == 0) tasks_s[e] = RC_REDO;
