@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # NB=900, maxrank 150, eps 1e-8), measured with two rocprofv3 --pmc passes
 # (FETCH_SIZE, WRITE_SIZE) summed over the group's kernels and scaled by the
 # counters' reading of a device copy of known size (x2.0 and x1.0).
-C2_TRAFFIC_BYTES_PER_GROUP = 6.559e9
+C2_TRAFFIC_BYTES_PER_GROUP = 6.399e9
 
 
 def log(*a):
