@@ -561,6 +561,75 @@ __global__ void k_classify_wave(TLR t, int k, int *tasks_s, int *wlist, int *wco
     }
 }
 
+// One-sided Jacobi on the rows of M (lane c owns row c of M in xm, zero
+// beyond s), round-robin pairs; a lane reads its partner's row with
+// ds_bpermute (no LDS storage, no wave barrier).  Loops cover the first W
+// entries (W >= s).  Returns ||M||_F; counts a run out of sweeps in *unconv.
+template <int SM, int W>
+__device__ __forceinline__ double wv_jacobi(double (&xm)[SM], int s, double acc, int fixed_rk, int *unconv)
+{
+    const int lane = threadIdx.x & 63;
+    const int npad = s + (s & 1);
+    const double tol = 4.0e-16 * sqrt((double)s);
+    double al = 0.0;
+#pragma unroll
+    for (int q = 0; q < W; q++) al = fma(xm[q], xm[q], al);
+    // A pair is left alone when its smaller column is below the
+    // rounding-noise floor (2.3e-16 s)^2 ||M||_F^2, or, when truncating
+    // at acc, below (1e-3 acc)^2: such columns stay below the
+    // truncation threshold (all of them together hold less than
+    // 0.012 acc of energy) and rotating them moves a kept column by
+    // less than one rounding of its norm.
+    const double f2 = wsum(lane < s ? al : 0.0);
+    const double nz = wv_skip_floor(f2, s, acc, fixed_rk);
+    // position of column `lane` in round 0 of the round-robin schedule
+    const int p0 = (lane == 0) ? 0 : lane;
+    int sweep = 0;
+    for (; sweep < BLK_JACOBI_SWEEPS; sweep++) {
+        int rot_any = 0;
+        int pos = p0;  // column at position p in round r: ((p - 1 + r) mod (npad - 1)) + 1
+        for (int r = 0; r < npad - 1; r++) {
+            const int pp = npad - 1 - pos;
+            int pc = 0;
+            if (pp != 0) {
+                pc = pp + r;
+                if (pc > npad - 1) pc -= npad - 1;
+            }
+            const int src = 4 * pc;  // pc < npad <= 64
+            double yv[W];
+#pragma unroll
+            for (int q = 0; q < W; q++) yv[q] = wv_bperm(src, xm[q]);
+            const double be = wv_bperm(src, al);
+            if (lane < s && pc < s) {
+                double g4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int q = 0; q < W; q++) g4[q & 3] = fma(xm[q], yv[q], g4[q & 3]);
+                const double ga = (g4[0] + g4[1]) + (g4[2] + g4[3]);
+                const bool lo_ = lane < pc;
+                double cs, sn;
+                if (fmin(al, be) > nz && jacobi_rot(lo_ ? al : be, lo_ ? be : al, ga, tol, cs, sn)) {
+                    rot_any = 1;
+                    const double ss = lo_ ? -sn : sn;
+#pragma unroll
+                    for (int q = 0; q < W; q++) xm[q] = fma(ss, yv[q], cs * xm[q]);
+                    // squared norm after the rotation
+                    al = lo_ ? (cs * cs * al - 2.0 * cs * sn * ga + sn * sn * be)
+                             : (sn * sn * be + 2.0 * cs * sn * ga + cs * cs * al);
+                }
+            }
+            // next round: every column but 0 moves one position down
+            if (lane > 0) pos = (pos == 1) ? npad - 1 : pos - 1;
+        }
+        // exact norms once per sweep
+        al = 0.0;
+#pragma unroll
+        for (int q = 0; q < W; q++) al = fma(xm[q], xm[q], al);
+        if (__ballot(rot_any) == 0) break;
+    }
+    if (sweep >= BLK_JACOBI_SWEEPS && lane == 0) atomicAdd(unconv, 1);
+    return sqrt(f2);
+}
+
 template <int SM>
 __global__ void __launch_bounds__(64 * WvCfg<SM>::WPB, WvCfg<SM>::MINW)
 k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *tasks_s, double *ws, double acc,
@@ -716,71 +785,16 @@ k_rc_wave(TLR t, int k, const int *wlist, const int *wcount, int *wnext, int *ta
             }
             wv_sync();
         }
-        // ---- one-sided Jacobi on the columns of M^T (lane c owns column c);
-        //      a lane reads its partner's column with ds_bpermute (no LDS
-        //      storage, no wave barrier); columns beyond s are zero, so every
-        //      loop runs over all SM entries without guards ----
-        double fro = 0.0;  // ||M||_F
-        {
-            const int npad = s + (s & 1);
-            const double tol = 4.0e-16 * sqrt((double)s);
-            double al = 0.0;
-#pragma unroll
-            for (int q = 0; q < SM; q++) al = fma(xm[q], xm[q], al);
-            // A pair is left alone when its smaller column is below the
-            // rounding-noise floor (2.3e-16 s)^2 ||M||_F^2, or, when truncating
-            // at acc, below (1e-3 acc)^2: such columns stay below the
-            // truncation threshold (all of them together hold less than
-            // 0.012 acc of energy) and rotating them moves a kept column by
-            // less than one rounding of its norm.
-            const double f2 = wsum(lane < s ? al : 0.0);
-            fro = sqrt(f2);
-            const double nz = wv_skip_floor(f2, s, acc, fixed_rk);
-            // position of column `lane` in round 0 of the round-robin schedule
-            const int p0 = (lane == 0) ? 0 : lane;
-            int sweep = 0;
-            for (; sweep < BLK_JACOBI_SWEEPS; sweep++) {
-                int rot_any = 0;
-                int pos = p0;  // column at position p in round r: ((p - 1 + r) mod (npad - 1)) + 1
-                for (int r = 0; r < npad - 1; r++) {
-                    const int pp = npad - 1 - pos;
-                    int pc = 0;
-                    if (pp != 0) {
-                        pc = pp + r;
-                        if (pc > npad - 1) pc -= npad - 1;
-                    }
-                    const int src = 4 * pc;  // pc < npad <= 64
-                    double yv[SM];
-#pragma unroll
-                    for (int q = 0; q < SM; q++) yv[q] = wv_bperm(src, xm[q]);
-                    const double be = wv_bperm(src, al);
-                    if (lane < s && pc < s) {
-                        double g4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                        for (int q = 0; q < SM; q++) g4[q & 3] = fma(xm[q], yv[q], g4[q & 3]);
-                        const double ga = (g4[0] + g4[1]) + (g4[2] + g4[3]);
-                        const bool lo_ = lane < pc;
-                        double cs, sn;
-                        if (fmin(al, be) > nz && jacobi_rot(lo_ ? al : be, lo_ ? be : al, ga, tol, cs, sn)) {
-                            rot_any = 1;
-                            const double ss = lo_ ? -sn : sn;
-#pragma unroll
-                            for (int q = 0; q < SM; q++) xm[q] = fma(ss, yv[q], cs * xm[q]);
-                            // squared norm after the rotation
-                            al = lo_ ? (cs * cs * al - 2.0 * cs * sn * ga + sn * sn * be)
-                                     : (sn * sn * be + 2.0 * cs * sn * ga + cs * cs * al);
-                        }
-                    }
-                    // next round: every column but 0 moves one position down
-                    if (lane > 0) pos = (pos == 1) ? npad - 1 : pos - 1;
-                }
-                // exact norms once per sweep
-                al = 0.0;
-#pragma unroll
-                for (int q = 0; q < SM; q++) al = fma(xm[q], xm[q], al);
-                if (__ballot(rot_any) == 0) break;
-            }
-            if (sweep >= BLK_JACOBI_SWEEPS && lane == 0) atomicAdd(o.unconv, 1);
+        // ---- one-sided Jacobi on the columns of M^T (lane c owns column c):
+        //      loops run over SM - 8 entries when s fits in them (entries
+        //      beyond s are zero, so the width changes no bit) ----
+        double fro;
+        if constexpr (SM <= 48) {
+            constexpr int WJ = SM - 8;
+            fro = (s <= WJ) ? wv_jacobi<SM, WJ>(xm, s, acc, fixed_rk, o.unconv)
+                            : wv_jacobi<SM, SM>(xm, s, acc, fixed_rk, o.unconv);
+        } else {  // one width: a second copy costs the widest class registers
+            fro = wv_jacobi<SM, SM>(xm, s, acc, fixed_rk, o.unconv);
         }
         // ---- singular values, order, rank ----
         double sig = 0.0;
