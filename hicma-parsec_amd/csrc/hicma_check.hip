@@ -74,12 +74,23 @@ __global__ void __launch_bounds__(256) k_dense_uncompress(TLR t, double *D, long
             }
             continue;
         }
-        const int r = *rank_at(t, m, n);
+        const int r = *rank_at(t, m, n), p = lr_prec(t, m, n);
         const double *U = lr_tile(t, m, n);
-        MkOps<false, false> op{U, NB, U + (size_t)NB * r, NB};
         mk_d4 acc[2][2];
         mk_zero(acc);
-        mk_accum(op, rows, cols, i0, j0, 0, r, acc, sm);
+        // U V^T from [U | V] in the type the tile is held in
+        if (p == LR_F64) {
+            const MkOps<false, false> op{U, NB, U + (size_t)NB * r, NB};
+            mk_accum(op, rows, cols, i0, j0, 0, r, acc, sm);
+        } else if (p == LR_F32) {
+            const float *Uf = (const float *)U;
+            const MkOps<false, false, float> op{Uf, NB, Uf + (size_t)NB * r, NB};
+            mk_accum(op, rows, cols, i0, j0, 0, r, acc, sm);
+        } else {
+            const lr_bf16 *Uh = (const lr_bf16 *)U;
+            const MkOps<false, false, lr_bf16> op{Uh, NB, Uh + (size_t)NB * r, NB};
+            mk_accum(op, rows, cols, i0, j0, 0, r, acc, sm);
+        }
         mk_store(acc, [&](int i, int j, double v) {
             if (i0 + i < rows && j0 + j < cols) dst[(i0 + i) + (long)(j0 + j) * ld] = v;
         });

@@ -148,6 +148,45 @@ __global__ void __launch_bounds__(256) k_solve_diag(TLR t, int k, const double *
     }
 }
 
+// The low-rank part of a solve update, [U | V] held as TE (widened on load,
+// FP64 arithmetic): w = F^T X_k with F = V (forward) or U (backward), a wave
+// per column; then P_o -= G w with G = U (forward) or V (backward).
+template <typename TE>
+__device__ __forceinline__ void solve_lr(const TE *T, int NB, int r, int trans, const double *xk, double *w,
+                                         double *Po, long ldp, int nq)
+{
+    const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
+    const TE *U = T, *V = T + (size_t)NB * r;
+    const TE *F = trans ? U : V, *Gm = trans ? V : U;
+    for (int c = g; c < r; c += 4) {
+        double acc[SV_RC];
+#pragma unroll
+        for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
+        for (int i = lane; i < NB; i += 64) {
+            const double fv = (double)F[i + (size_t)c * NB];
+#pragma unroll
+            for (int q = 0; q < SV_RC; q++) acc[q] = fma(fv, xk[i * SV_RC + q], acc[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < SV_RC; q++) {
+            const double v = wsum(acc[q]);
+            if (lane == 0) w[c * SV_RC + q] = v;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < NB; i += 256) {
+        double acc[SV_RC];
+#pragma unroll
+        for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
+        for (int c = 0; c < r; c++) {
+            const double gv = (double)Gm[i + (size_t)c * NB];
+#pragma unroll
+            for (int q = 0; q < SV_RC; q++) acc[q] = fma(gv, w[c * SV_RC + q], acc[q]);
+        }
+        for (int q = 0; q < nq; q++) Po[i + (size_t)q * ldp] -= acc[q];
+    }
+}
+
 // The update of one block of right-hand sides by the solved block k:
 // forward: P_m -= L_mk X_k for m = k + 1 + blockIdx.x; backward:
 // P_j -= L_kj^T X_k for j = blockIdx.x.  LDS: X_k (NB x SV_RC), w (cap x SV_RC).
@@ -165,7 +204,6 @@ __global__ void __launch_bounds__(256) k_solve_update(TLR t, int k, double *P, l
     const int r = dense ? 0 : *rank_at(t, m, n);
     if (!dense && r <= 0) return;
     const double *T = dense ? band_tile(t, m, n) : lr_tile(t, m, n);
-    const double *U = T, *V = T + (size_t)NB * r;
     const double *Pk = P + (size_t)k * NB;
     double *Po = P + (size_t)o * NB;
     for (int r0 = 0; r0 < nrhs; r0 += SV_RC) {
@@ -206,36 +244,10 @@ __global__ void __launch_bounds__(256) k_solve_update(TLR t, int k, double *P, l
                 }
             }
         } else {
-            // low rank: w = F^T X_k with F = V (forward) or U (backward), a wave
-            // per column; then P_o -= G w with G = U (forward) or V (backward)
-            const double *F = trans ? U : V, *Gm = trans ? V : U;
-            for (int c = g; c < r; c += 4) {
-                double acc[SV_RC];
-#pragma unroll
-                for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
-                for (int i = lane; i < NB; i += 64) {
-                    const double fv = F[i + (size_t)c * NB];
-#pragma unroll
-                    for (int q = 0; q < SV_RC; q++) acc[q] = fma(fv, xk[i * SV_RC + q], acc[q]);
-                }
-#pragma unroll
-                for (int q = 0; q < SV_RC; q++) {
-                    const double v = wsum(acc[q]);
-                    if (lane == 0) w[c * SV_RC + q] = v;
-                }
-            }
-            __syncthreads();
-            for (int i = tid; i < NB; i += 256) {
-                double acc[SV_RC];
-#pragma unroll
-                for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
-                for (int c = 0; c < r; c++) {
-                    const double gv = Gm[i + (size_t)c * NB];
-#pragma unroll
-                    for (int q = 0; q < SV_RC; q++) acc[q] = fma(gv, w[c * SV_RC + q], acc[q]);
-                }
-                for (int q = 0; q < nq; q++) Po[i + (size_t)(r0 + q) * ldp] -= acc[q];
-            }
+            const int p = lr_prec(t, m, n);
+            if (p == LR_F64) solve_lr(T, NB, r, trans, xk, w, Po + (size_t)r0 * ldp, ldp, nq);
+            else if (p == LR_F32) solve_lr((const float *)T, NB, r, trans, xk, w, Po + (size_t)r0 * ldp, ldp, nq);
+            else solve_lr((const lr_bf16 *)T, NB, r, trans, xk, w, Po + (size_t)r0 * ldp, ldp, nq);
         }
         __syncthreads();
     }

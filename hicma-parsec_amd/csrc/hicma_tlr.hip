@@ -247,12 +247,14 @@ extern "C" int hicma_desc_create(hicma_ctx *ctx, hicma_tlr_desc **out, int N, in
     size_t lr_slots = (size_t)NT * (NT - 1) / 2;
     if (ctx_malloc(ctx, (void **)&dx, sizeof(double) * (size_t)t.ndim * N) != hipSuccess ||
         ctx_malloc(ctx, (void **)&t.rank, sizeof(int) * (size_t)NT * NT) != hipSuccess ||
+        ctx_malloc(ctx, (void **)&t.prec, (size_t)NT * NT) != hipSuccess ||
         (lr_slots && ctx_malloc(ctx, (void **)&t.lr, sizeof(double) * lr_slots * NB * 2 * maxrank) != hipSuccess) ||
         ctx_malloc(ctx, (void **)&A->d_tileops, sizeof(unsigned long long) * (size_t)NT * NT) != hipSuccess ||
         ctx_malloc(ctx, (void **)&A->d_opstat, sizeof(unsigned long long) * 16) != hipSuccess) {
         (void)hipGetLastError();
         ctx_free(ctx, dx);
         ctx_free(ctx, t.rank);
+        ctx_free(ctx, t.prec);
         ctx_free(ctx, t.lr);
         ctx_free(ctx, A->d_tileops);
         free(A->h_xyz);
@@ -263,6 +265,7 @@ extern "C" int hicma_desc_create(hicma_ctx *ctx, hicma_tlr_desc **out, int N, in
     HIPCHK(hipMemcpy(dx, A->h_xyz, sizeof(double) * (size_t)t.ndim * N, hipMemcpyHostToDevice));
     t.xyz = dx;
     HIPCHK(hipMemsetAsync(t.rank, 0xFF, sizeof(int) * (size_t)NT * NT, ctx->stream));
+    HIPCHK(hipMemsetAsync(t.prec, LR_F64, (size_t)NT * NT, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     *out = A;
     return 0;
@@ -275,6 +278,7 @@ extern "C" int hicma_desc_destroy(hicma_tlr_desc *A)
     hipSetDevice(c->device);
     ctx_free(c, (void *)A->t.xyz);
     ctx_free(c, A->t.rank);
+    ctx_free(c, A->t.prec);
     ctx_free(c, A->t.lr);
     ctx_free(c, A->t.band);
     ctx_free(c, A->d_tileops);
@@ -390,6 +394,8 @@ extern "C" int STARSH_gen(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr
     int rc = alloc_band(A, band_size);
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(t.rank, 0xFF, sizeof(int) * (size_t)t.NT * t.NT, ctx->stream));
+    HIPCHK(hipMemsetAsync(t.prec, LR_F64, (size_t)t.NT * t.NT, ctx->stream));
+    A->demoted = 0;
     HIPCHK(hipMemsetAsync(ctx->d_info, 0, 4 * sizeof(int), ctx->stream));
     if ((rc = gen_band_tiles(A))) return rc;
     int NB = t.NB, p = maxrank + 10;
@@ -774,6 +780,8 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
     if (maxrank > compmaxrank) return HICMA_ERR_MAXRANK_COMP;
     if (maxrank > A->t.NB / 2) return HICMA_ERR_MAXRANK_NB;
     if (maxrank > A->t.cap || band_size != A->t.band_size) return HICMA_ERR_ARG;
+    // the factorization's kernels read and write FP64 tiles only
+    if (A->demoted) return HICMA_ERR_ARG;
     if (lookahead && *lookahead == -1) *lookahead = band_size;
     // 3-flow when band == 1 and not forced 2-flow, else 2-flow (hicma_parsec.c:517, 560-573)
     bool three_flow = (band_size == 1 && !(two_flow && *two_flow));
@@ -1133,6 +1141,8 @@ extern "C" int hicma_desc_copy(hicma_tlr_desc *dst, const hicma_tlr_desc *src)
         HIPCHK(hipMemcpyAsync(b.lr, a.lr, sizeof(double) * slots * a.NB * 2 * a.cap, hipMemcpyDeviceToDevice,
                               c->stream));
     HIPCHK(hipMemcpyAsync(b.rank, a.rank, sizeof(int) * (size_t)a.NT * a.NT, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b.prec, a.prec, (size_t)a.NT * a.NT, hipMemcpyDeviceToDevice, c->stream));
+    dst->demoted = src->demoted;
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -1272,7 +1282,26 @@ extern "C" int hicma_get_tile(const hicma_tlr_desc *A, int m, int n, double *buf
     if (rank) *rank = r;
     if (r > 0) {
         const double *src = t.lr + ((size_t)m * (m - 1) / 2 + n) * (size_t)t.NB * 2 * t.cap;
-        HIPCHK(hipMemcpy(buf, src, sizeof(double) * (size_t)t.NB * 2 * r, hipMemcpyDeviceToHost));
+        unsigned char p = LR_F64;
+        HIPCHK(hipMemcpy(&p, t.prec + (size_t)n * t.NT + m, 1, hipMemcpyDeviceToHost));
+        const size_t ne = (size_t)t.NB * 2 * r, es = p == LR_F64 ? 8 : (p == LR_F32 ? 4 : 2);
+        HIPCHK(hipMemcpy(buf, src, es * ne, hipMemcpyDeviceToHost));
+        // widen in place from the last element down: double i covers packed
+        // elements 2i.. (FP32) or 4i.. (BF16), none below i, so every packed
+        // element it replaces has been widened already
+        char *raw = (char *)buf;
+        for (size_t i = ne; es != 8 && i-- > 0;) {
+            float f;
+            if (es == 4) {
+                memcpy(&f, raw + 4 * i, 4);
+            } else {
+                unsigned short h;
+                memcpy(&h, raw + 2 * i, 2);
+                const unsigned w = (unsigned)h << 16;
+                memcpy(&f, &w, 4);
+            }
+            buf[i] = (double)f;
+        }
     }
     return 0;
 }
@@ -1294,6 +1323,14 @@ extern "C" int hicma_set_tile(hicma_tlr_desc *A, int m, int n, const double *buf
     double *dst = t.lr + ((size_t)m * (m - 1) / 2 + n) * (size_t)t.NB * 2 * t.cap;
     if (rank > 0) HIPCHK(hipMemcpy(dst, buf, sizeof(double) * (size_t)t.NB * 2 * rank, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(t.rank + (size_t)n * t.NT + m, &rank, sizeof(int), hipMemcpyHostToDevice));
+    // the tile now holds FP64 values
+    unsigned char *pp = t.prec + (size_t)n * t.NT + m, p = LR_F64;
+    HIPCHK(hipMemcpy(&p, pp, 1, hipMemcpyDeviceToHost));
+    if (p != LR_F64) {
+        const unsigned char f64 = LR_F64;
+        HIPCHK(hipMemcpy(pp, &f64, 1, hipMemcpyHostToDevice));
+        A->demoted--;
+    }
     return 0;
 }
 

@@ -78,6 +78,7 @@ struct hicma_tlr_desc {
     int last_unconv;
     int *rhist;                     // NT*NT x {initial, min, max, final} rank history, or null
     int rhist_on;
+    int demoted;                    // off-band tiles held in FP32 or BF16 (hicma_tlr_demote)
     int use_graph;                  // capture the factorization once, replay it after
     hipGraph_t graph;
     hipGraphExec_t gexec;

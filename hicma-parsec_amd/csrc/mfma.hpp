@@ -26,12 +26,13 @@ __device__ __forceinline__ mk_d4 mk_mfma(double a, double b, mk_d4 c)
 
 // Operand addressing.  A element (i, k): AK ? A[k + i lda] : A[i + k lda]
 // (AK = "k contiguous", i.e. A given transposed).  B element (k, j):
-// BK ? B[k + j ldb] : B[j + k ldb].
-template <bool AK, bool BK>
+// BK ? B[k + j ldb] : B[j + k ldb].  TE: the element type in memory, widened
+// to double on load (double, float, or a type that converts to double).
+template <bool AK, bool BK, typename TE = double>
 struct MkOps {
-    const double *A;
+    const TE *A;
     long lda;
-    const double *B;
+    const TE *B;
     long ldb;
 };
 
@@ -40,8 +41,8 @@ struct MkOps {
 // their (row, k) / (column, k) indices, A(i, k) = 0 for k > i and
 // B(k, j) = 0 for k > j, whatever memory holds there.  All 256 threads must
 // call it uniformly.
-template <bool AK, bool BK, bool TRI = false>
-__device__ __forceinline__ void mk_accum(const MkOps<AK, BK> &op, int M, int N, int i0, int j0, long k0, long k1,
+template <bool AK, bool BK, bool TRI = false, typename TE = double>
+__device__ __forceinline__ void mk_accum(const MkOps<AK, BK, TE> &op, int M, int N, int i0, int j0, long k0, long k1,
                                          mk_d4 (&acc)[2][2], double *sm)
 {
     double *As = sm, *Bs = sm + MK_KC * MK_LD;
@@ -59,15 +60,16 @@ __device__ __forceinline__ void mk_accum(const MkOps<AK, BK> &op, int M, int N, 
             else { ii = e & (MK_T - 1); kk = e >> 6; }
             const long gk = kc + kk;
             const int gi = i0 + ii;
-            pa[q] = (gi < M && gk < k1 && (!TRI || gk <= gi)) ? (AK ? op.A[gk + (long)gi * op.lda] : op.A[gi + gk * op.lda])
-                                                               : 0.0;
+            pa[q] = (gi < M && gk < k1 && (!TRI || gk <= gi))
+                        ? (double)(AK ? op.A[gk + (long)gi * op.lda] : op.A[gi + gk * op.lda])
+                        : 0.0;
             int jj;
             if (BK) { kk = e & (MK_KC - 1); jj = e >> 4; }
             else { jj = e & (MK_T - 1); kk = e >> 6; }
             const long gk2 = kc + kk;
             const int gj = j0 + jj;
             pb[q] = (gj < N && gk2 < k1 && (!TRI || gk2 <= gj))
-                        ? (BK ? op.B[gk2 + (long)gj * op.ldb] : op.B[gj + gk2 * op.ldb])
+                        ? (double)(BK ? op.B[gk2 + (long)gj * op.ldb] : op.B[gj + gk2 * op.ldb])
                         : 0.0;
         }
     };

@@ -7,6 +7,10 @@
 //   off-band tile (m,n), m > n   : slot of NB*2*cap doubles at lr + (m(m-1)/2+n)*NB*2*cap,
 //                                  compact [U | V]: U = NB x r, V = NB x r at offset NB*r
 //   rank map                     : int32 NT x NT column major, rank[n*NT + m]
+//   precision map                : uint8 NT x NT column major, prec[n*NT + m]: the
+//                                  element type the off-band tile's [U | V] is held in
+//                                  (LR_F64, or after hicma_tlr_demote LR_F32 / LR_BF16,
+//                                  packed from the start of the slot, V at element NB*r)
 // The matrix is padded to NT*NB with an identity block, so every tile is
 // NB x NB and the factor of the padding is the identity (log 1 = 0).
 #pragma once
@@ -17,6 +21,7 @@ struct TLR {
     double *band;
     double *lr;
     int *rank;
+    unsigned char *prec;
     const double *xyz;
     int N, NB, NT, band_size, cap, kind, ndim;
     double sigma2, beta, nugget;
@@ -34,6 +39,17 @@ __device__ __forceinline__ int *rank_at(const TLR &t, int m, int n)
 {
     return t.rank + (size_t)n * t.NT + m;
 }
+// element types of an off-band tile's [U | V]
+#define LR_F64 0
+#define LR_F32 1
+#define LR_BF16 2
+// a bfloat16 element: the upper half of an IEEE binary32
+struct lr_bf16 {
+    unsigned short u;
+    __device__ __forceinline__ operator double() const { return (double)__uint_as_float((unsigned)u << 16); }
+};
+__device__ __forceinline__ int lr_prec(const TLR &t, int m, int n) { return t.prec[(size_t)n * t.NT + m]; }
+
 __device__ __forceinline__ int tdim(const TLR &t, int m) { return m == t.NT - 1 ? t.N - m * t.NB : t.NB; }
 
 // covariance entry for global indices (identity on the padding)

@@ -37,6 +37,7 @@ EXPORTS = [
     "hicma_tlr_loglik", "hicma_dev_malloc", "hicma_dev_free", "hicma_memcpy", "hicma_tlr_loglik_noalloc",
     "hicma_init_ex", "hicma_server_create", "hicma_server_submit", "hicma_server_wait", "hicma_server_destroy",
     "hicma_normal", "hicma_factor_graph", "hicma_set_wave_min_tasks", "hicma_autotune_schedule",
+    "hicma_tlr_normf", "hicma_tlr_demote",
 ]
 
 _lib = None
@@ -129,6 +130,8 @@ def lib():
             "hicma_factor_graph": (i, [vp, i]),
             "hicma_set_wave_min_tasks": (i, [vp, i]),
             "hicma_autotune_schedule": (i, [vp, vp, d, i, i, C.POINTER(i), C.POINTER(d)]),
+            "hicma_tlr_normf": (i, [vp, C.POINTER(d), dp]),
+            "hicma_tlr_demote": (i, [vp, d, d, ip]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -530,6 +533,20 @@ class TLRMatrix:
         v = C.c_double()
         _check(lib().hicma_logdet(self.h, C.byref(v)), "hicma_logdet")
         return v.value
+
+    def normf(self, tiles=False):
+        """||A||_F of the symmetric matrix held (and the lower tiles' norms, NT x NT)."""
+        v = C.c_double()
+        T = np.zeros(self.NT * self.NT) if tiles else None
+        _check(lib().hicma_tlr_normf(self.h, C.byref(v), T.ctypes.data if tiles else None), "hicma_tlr_normf")
+        return (v.value, T.reshape(self.NT, self.NT, order="F")) if tiles else v.value
+
+    def demote(self, thr32, thr16=0.0):
+        """Hold off-band tiles with norm < thr32 in FP32, < thr16 in BF16;
+        returns the tile counts (FP64, FP32, BF16)."""
+        c = np.zeros(3, dtype=np.int32)
+        _check(lib().hicma_tlr_demote(self.h, thr32, thr16, c.ctypes.data), "hicma_tlr_demote")
+        return tuple(int(x) for x in c)
 
     def uncompress(self):
         """Host copy of the uncompressed lower triangle (STARSH_check into a

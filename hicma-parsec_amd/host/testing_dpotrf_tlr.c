@@ -47,6 +47,8 @@ static void usage(void)
             " -Y --lookahead, -W --two-flow, -F --send_full_tile, -z --HNB, -c --cores, -P, -Q\n"
             " -x --check    : dense checks on the device (check_diff, check_dpotrf2, log-det)\n"
             " --print-rank  : print the initial/min/max/final rank maps\n"
+            " --mixed       : after the factorization, hold off-band tiles with ||L_mn||_F < eps ||A||_F in FP32\n"
+            "                 (< 1e-3 eps ||A||_F in BF16); --check then checks that factor\n"
             " -v --verbose\n"
             " --beta, --nugget, --sigma2, --seed : kernel parameters\n");
 }
@@ -55,7 +57,7 @@ int main(int argc, char **argv)
 {
     int N = 0, NB = 0, fixedrk = 0, maxrank = 0, genmaxrank = 0, compmaxrank = 0, kind = 2;
     int band_size = 1, auto_band = 1, lookahead = -1, two_flow = 0, send_full_tile = 0, HNB = 300;
-    int cores = 1, P = 1, Q = 1, check = 0, verbose = 0, print_rank = 0;
+    int cores = 1, P = 1, Q = 1, check = 0, verbose = 0, print_rank = 0, mixed = 0;
     double tol = 1e-8, beta = 0.1, nugget = 0.1, sigma2 = 1.0;
     unsigned long long seed = 0;
     static struct option lo[] = {
@@ -80,7 +82,7 @@ int main(int argc, char **argv)
         {"W", required_argument, 0, 'W'},        {"beta", required_argument, 0, 1001},
         {"nugget", required_argument, 0, 1002},  {"sigma2", required_argument, 0, 1003},
         {"seed", required_argument, 0, 1004},    {"print-rank", no_argument, 0, 1005},
-        {0, 0, 0, 0}};
+        {"mixed", no_argument, 0, 1006},         {0, 0, 0, 0}};
     int c, opt = 0;
     while ((c = getopt_long_only(argc, argv, "", lo, &opt)) != -1) {
         switch (c) {
@@ -108,6 +110,7 @@ int main(int argc, char **argv)
         case 1003: sigma2 = atof(optarg); break;
         case 1004: seed = strtoull(optarg, NULL, 10); break;
         case 1005: print_rank = 1; break;
+        case 1006: mixed = 1; break;
         case 'h': usage(); return 0;
         default: usage(); return 1;
         }
@@ -226,6 +229,8 @@ int main(int argc, char **argv)
         }
     }
     if (print_rank) hicma_rank_history(A, 1);
+    double normA = 0;
+    if (mixed && hicma_tlr_normf(A, &normA, NULL)) ret |= 1;
 
     unsigned long *tileops = (unsigned long *)calloc((size_t)NT * NT, sizeof(unsigned long));
     unsigned long *opc = (unsigned long *)calloc((size_t)(cores > 4 ? cores : 4), sizeof(unsigned long));
@@ -244,6 +249,15 @@ int main(int argc, char **argv)
         ret |= 1;
     }
     if (print_rank) parsec_rank_print(ctx, A, band_size);
+    if (mixed) {
+        /* mixed-precision storage of the factor; the checks below run on it */
+        int cnt[3] = {0, 0, 0};
+        t0 = wtime();
+        if (hicma_tlr_demote(A, tol * normA, 1e-3 * tol * normA, cnt)) ret |= 1;
+        printf("[****] TIME(s) %12.5f : Mixed_precision\t||A||_F= %e thresholds %e %e tiles FP64= %d FP32= %d "
+               "BF16= %d\n",
+               wtime() - t0, normA, tol * normA, 1e-3 * tol * normA, cnt[0], cnt[1], cnt[2]);
+    }
     if (check) {
         printf("\nCHECK RESULT:\n\n");
         /* check_diff: L of the TLR factorization against the dense factor of the approximate matrix */

@@ -230,7 +230,8 @@ int hicma_check_diff(hicma_ctx *ctx, int verbose, int uplo, const hicma_dense *L
                      double threshold, double *result);
 
 /* ---- tile access (host copies) ---- */
-/* band tile: NB*NB doubles, returns NB in *rank; off-band: NB*2*r doubles */
+/* band tile: NB*NB doubles, returns NB in *rank; off-band: NB*2*r doubles
+ * (a tile held in FP32 or BF16 is widened to doubles) */
 int hicma_get_tile(const hicma_tlr_desc *A, int m, int n, double *buf, int *rank);
 int hicma_set_tile(hicma_tlr_desc *A, int m, int n, const double *buf, int rank);
 /* whole rank map, NT*NT, as stored on the device */
@@ -250,6 +251,25 @@ int hicma_tlr_potrs(hicma_ctx *ctx, hicma_tlr_desc *A, int nrhs, double *B, int 
  * logdet = log det Sigma, quad = z^T Sigma^-1 z,
  * loglik = -(quad + logdet + N log 2 pi) / 2; any output may be NULL */
 int hicma_tlr_loglik(hicma_ctx *ctx, hicma_tlr_desc *A, double *z, double *logdet, double *quad, double *loglik);
+
+/* ---- mixed-precision storage of the factor ----
+ * No counterpart in the reference, which holds every tile in FP64.  The
+ * factorization computes in FP64; afterwards the off-band tiles whose
+ * Frobenius norm is small can be held in FP32 or BF16, and the solves, the
+ * log-likelihood, STARSH_check and hicma_get_tile read them in that type.
+ * hicma_tlr_normf: ||A||_F of the symmetric matrix the descriptor holds
+ * (lower tiles counted twice, the padding left out) and, if tile_norms is
+ * not NULL, the Frobenius norm of every lower tile (host, NT*NT doubles,
+ * column major, zero above the diagonal).  */
+int hicma_tlr_normf(hicma_tlr_desc *A, double *normF, double *tile_norms);
+/* Round in place every off-band tile with ||L_mn||_F < thr16 to BF16 and
+ * every other one with ||L_mn||_F < thr32 to FP32 (thr16 <= thr32; thr16 <= 0
+ * disables BF16); with thr32 = eps ||A||_F the rounding error of each tile is
+ * below 2^-24 eps ||A||_F.  counts (may be NULL): off-band tiles now held in
+ * FP64, FP32, BF16.  HiCMA_dpotrf_L refuses a descriptor holding demoted
+ * tiles (HICMA_ERR_ARG); STARSH_gen, and hicma_set_tile for its tile, make
+ * tiles FP64 again. */
+int hicma_tlr_demote(hicma_tlr_desc *A, double thr32, double thr16, int *counts);
 
 /* hicma_tlr_loglik without any device allocation.  No counterpart in the
  * reference: for callers that cannot allocate (code under HIP graph capture,
