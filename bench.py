@@ -180,6 +180,44 @@ def serving_sweep(rates, nreq, slots, budget_gb, seed=2024):
     return rows
 
 
+def mixed_precision(A, P, eps, reps=5):
+    """The factor A held in mixed precision (off-band tiles with ||L_mn||_F <
+    eps ||A||_F in FP32, < 1e-3 eps ||A||_F in BF16; P holds the matrix before
+    the factorization): tile counts, off-band bytes held, and the
+    log-likelihood of one seeded z on the FP64 and on the mixed factor (time
+    of the device solve, median of reps; difference of z^T Sigma^-1 z)."""
+    normA = P.normf()
+    z = np.random.default_rng(11).standard_normal(A.N)
+
+    def timed():
+        ts, r = [], None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            r = A.loglik(z)
+            ts.append(time.perf_counter() - t0)
+        return r, float(np.median(ts))
+
+    (ld64, q64, _), t64 = timed()
+    _, T = A.normf(tiles=True)
+    R = A.rank_map()
+    thr32, thr16 = eps * normA, 1e-3 * eps * normA
+    b64 = bmx = 0
+    for m in range(A.NT):
+        for n in range(m - A.band + 1):
+            r = max(int(R[m, n]), 0)
+            b = A.NB * 2 * r
+            b64 += 8 * b
+            bmx += b * (2 if T[m, n] < thr16 else (4 if T[m, n] < thr32 else 8))
+    t0 = time.perf_counter()
+    counts = A.demote(thr32, thr16)
+    tdem = time.perf_counter() - t0
+    (ld, q, _), tmx = timed()
+    return {"normA_F": normA, "thresholds": [thr32, thr16], "tiles_f64_f32_bf16": list(counts),
+            "offband_bytes_f64": b64, "offband_bytes_mixed": bmx, "demote_s": round(tdem, 4),
+            "loglik_s_f64": round(t64, 5), "loglik_s_mixed": round(tmx, 5),
+            "logdet_identical": bool(ld == ld64), "quad_rel_diff": abs(q - q64) / abs(q64)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -263,6 +301,12 @@ def main():
     log(f"final ranks min/avg/max {fmin}/{favg:.1f}/{fmax}; logdet {logdet:.12e}; overflows {A.rank_overflows()}")
     assert np.isfinite(logdet) and A.rank_overflows() == 0
 
+    mixed = mixed_precision(A, pristine, a.eps)
+    log(f"mixed precision: tiles FP64/FP32/BF16 {mixed['tiles_f64_f32_bf16']}, off-band bytes "
+        f"{mixed['offband_bytes_f64'] / 1e9:.2f} -> {mixed['offband_bytes_mixed'] / 1e9:.2f} GB, log-likelihood "
+        f"{mixed['loglik_s_f64'] * 1e3:.2f} -> {mixed['loglik_s_mixed'] * 1e3:.2f} ms, z^T Sigma^-1 z relative "
+        f"difference {mixed['quad_rel_diff']:.2e}")
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(pristine, total_ops, a.cpu_seconds)
@@ -323,6 +367,7 @@ def main():
             "algorithmic_GBs": round(rc_bytes / kernel_s * 1e-9, 2),
         },
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
         "serving": serving,
     }
     print(json.dumps(out))
