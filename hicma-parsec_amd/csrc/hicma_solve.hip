@@ -3,29 +3,50 @@
 //
 // No counterpart in the reference (it stops at the factor); the solves read
 // the factor in the layout HiCMA_dpotrf_L leaves (dense band tiles, compact
-// [U | V] off-band tiles).  Blocked substitution by tile columns:
+// [U | V] off-band tiles, each in the type the precision map gives).
+// Blocked substitution by tile columns:
 //   forward  (B <- L^-1 B):  X_k = L_kk^-1 B_k, then B_m -= L_mk X_k  (m > k)
 //   backward (B <- L^-T B):  X_k = L_kk^-T B_k, then B_j -= L_kj^T X_k (j < k)
-// with L_mk X_k = U (V^T X_k) for low-rank tiles.  Each diagonal solve uses
-// the inverses of the 64 x 64 diagonal blocks of L_kk, formed once per call
-// for all tiles in one launch (one wave per block), so a tile's solve is a
-// sequence of small matrix-vector products.  All sums run in a fixed order:
-// the results do not depend on scheduling.
+// with L_mk X_k = U (V^T X_k) for low-rank tiles.  Every L_kk^-1 is formed
+// once per call (the 64 x 64 diagonal-block inverses one wave each, the
+// blocks below on the matrix cores), so the diagonal step is a block
+// matrix-vector product over 64-row blocks in parallel instead of a chain
+// of dependent 64-row substitutions, and each update runs one workgroup per
+// 64-row block of each tile.  The NT diagonal/update launch pairs of a pass
+// remain a chain.  A pass reads its right-hand sides from one buffer and
+// writes its solution to another, so no launch updates in place what another
+// workgroup of it still reads.  All sums run in a fixed order: the results do
+// not depend on scheduling.
 #include "internal.hpp"
+#include "mfma.hpp"
 #include "wave.hpp"
 
 #include <math.h>
+#include <utility>
 
-#define SV_RC 4   // right-hand sides per pass (one wave per right-hand side in the block products)
-#define SV_BK 64  // diagonal block edge
-#define SV_WL (SV_BK + 1)
+#define SV_RC 4                // right-hand sides per pass
+#define SV_BK 64               // diagonal block edge
+#define SU_T 1024              // threads of the diagonal and update kernels
+#define SU_G (SU_T / SV_BK)    // their column groups (one wave each)
+
+// L_kk^-1 is held as nb packed block-column panels per diagonal tile: panel
+// c holds block rows c .. nb-1 of block column c, column major with
+// ld 64 (nb - c); the panels of tile k start at Xinv + k inv_tile(nb).
+__host__ __device__ __forceinline__ size_t inv_tile(int nb) { return (size_t)SV_BK * SV_BK * nb * (nb + 1) / 2; }
+__host__ __device__ __forceinline__ size_t inv_poff(int nb, int c)
+{
+    return (size_t)SV_BK * SV_BK * ((size_t)c * nb - (size_t)c * (c - 1) / 2);
+}
 
 // W = L_bb^-1 for every 64 x 64 diagonal block b of every diagonal tile k,
-// row-major W(i, c) at Winv[((k nbk + b) 64 + i) 64 + c]; rows and columns
-// beyond the tile (NB not a multiple of 64) hold the identity.
-__global__ void __launch_bounds__(64) k_tri_inv(TLR t, double *Winv)
+// into block 0 of panel b (rows and columns beyond the tile, NB not a
+// multiple of 64, hold the identity).  One wave per block: lane c forms
+// column c of W by forward substitution, W(i, c) = (e_c(i) - sum_{c <= l < i}
+// L(i, l) W(l, c)) / L(i, i), with L and W in LDS.
+__global__ void __launch_bounds__(64) k_tri_inv(TLR t, double *Xinv)
 {
-    __shared__ double Lr[SV_BK * (SV_BK + 2)];
+    __shared__ double Lr[SV_BK * (SV_BK + 1)];  // L(i, l) at Lr[i * 65 + l]
+    __shared__ double Ws[SV_BK * (SV_BK + 1)];  // W(i, c) at Ws[i * 65 + c]
     const int NB = t.NB, nbk = (NB + SV_BK - 1) / SV_BK;
     const int k = blockIdx.x / nbk, b = blockIdx.x % nbk, j0 = SV_BK * b;
     const int jb = min(SV_BK, NB - j0), lane = threadIdx.x;
@@ -33,139 +54,154 @@ __global__ void __launch_bounds__(64) k_tri_inv(TLR t, double *Winv)
     for (int c = 0; c < SV_BK; c++) {
         double v = (lane == c) ? 1.0 : 0.0;
         if (lane < jb && c < jb) v = (c <= lane) ? Lk[lane + (size_t)c * NB] : 0.0;
-        Lr[lane * (SV_BK + 2) + c] = v;
+        Lr[lane * (SV_BK + 1) + c] = v;
     }
     wv_sync();
-    // lane = column c of W: right-looking substitution L w = e_c
-    double a[SV_BK];
-#pragma unroll
-    for (int i = 0; i < SV_BK; i++) a[i] = (i == lane) ? 1.0 : 0.0;
-#pragma unroll
-    for (int l = 0; l < SV_BK; l++) {
-        const double w = (lane <= l) ? a[l] / Lr[l * (SV_BK + 2) + l] : 0.0;
-        a[l] = w;
-#pragma unroll
-        for (int i = l + 1; i < SV_BK; i++) a[i] = fma(-Lr[i * (SV_BK + 2) + l], w, a[i]);
+    const int c = lane;
+    for (int i = 0; i < SV_BK; i++) {
+        double s = (i == c) ? 1.0 : 0.0;
+        for (int l = c; l < i; l++) s = fma(-Lr[i * (SV_BK + 1) + l], Ws[l * (SV_BK + 1) + c], s);
+        Ws[i * (SV_BK + 1) + c] = (i >= c) ? s / Lr[i * (SV_BK + 1) + i] : 0.0;
     }
-    double *W = Winv + (size_t)blockIdx.x * SV_BK * SV_BK;
-#pragma unroll
-    for (int i = 0; i < SV_BK; i++) W[i * SV_BK + lane] = a[i];
+    wv_sync();
+    const size_t ld = (size_t)SV_BK * (nbk - b);
+    double *W = Xinv + (size_t)k * inv_tile(nbk) + inv_poff(nbk, b);
+    for (int j = 0; j < SV_BK; j++) W[lane + (size_t)j * ld] = Ws[lane * (SV_BK + 1) + j];
 }
 
-// Solve with the diagonal tile k in place on rows k NB .. k NB + NB - 1 of
-// the padded right-hand sides P (ld ldp).  One workgroup of 256 threads.
-// LDS: x (NBp x SV_RC), W block (64 x 65), partial sums (4 x 64 x SV_RC).
-__global__ void __launch_bounds__(256) k_solve_diag(TLR t, int k, const double *Winv, double *P, long ldp, int nrhs,
-                                                    int trans)
+// The off-diagonal blocks of L_kk^-1, block column c of tile k per
+// workgroup (grid NT nb): X_bc = -W_b sum_{j=c}^{b-1} L_bj X_jc for
+// b = c+1 .. nb-1, each sum one matrix-core product of the row panel
+// L(b, c .. b-1) with the panel's blocks c .. b-1, stored into block b's
+// slot, then multiplied by W_b from panel b.
+__global__ void __launch_bounds__(256) k_tile_inv(TLR t, double *Xinv)
+{
+    __shared__ double sm[MK_SMEM];
+    const int NB = t.NB, nb = (NB + SV_BK - 1) / SV_BK;
+    const int k = blockIdx.x / nb, c = blockIdx.x % nb;
+    const int ldx = SV_BK * (nb - c);
+    double *X0 = Xinv + (size_t)k * inv_tile(nb);
+    double *Xp = X0 + inv_poff(nb, c);
+    const double *Lk = band_tile(t, k, k);
+    for (int b = c + 1; b < nb; b++) {
+        mk_d4 acc[2][2];
+        mk_zero(acc);
+        // S = L(64b .., 64c .. 64b) Xp(0 .. 64(b - c), :)
+        const MkOps<false, true> op{Lk + (size_t)SV_BK * c * NB, NB, Xp, ldx};
+        mk_accum(op, NB, SV_BK, SV_BK * b, 0, 0, SV_BK * (b - c), acc, sm);
+        double *Sb = Xp + SV_BK * (b - c);
+        mk_store(acc, [&](int i, int j, double v) { Sb[i + (size_t)j * ldx] = v; });
+        __syncthreads();
+        // X_bc = -W_b S; W_b is block 0 of panel b
+        mk_zero(acc);
+        const MkOps<false, true> op2{X0 + inv_poff(nb, b), SV_BK * (nb - b), Sb, ldx};
+        mk_accum(op2, SV_BK, SV_BK, 0, 0, 0, SV_BK, acc, sm);  // ends with a barrier: S is read
+        mk_store(acc, [&](int i, int j, double v) { Sb[i + (size_t)j * ldx] = -v; });
+        __syncthreads();
+    }
+}
+
+// x_k = L_kk^-1 y_k (forward) or L_kk^-T y_k (backward), block row b per
+// workgroup (grid nb): y from the rows of tile k in Pin, x into Pout.
+// forward:  x_b = sum_{c <= b} X_bc y_c, thread (i, g) takes columns g + 16u
+//           of each block, the 16 groups summed in order;
+// backward: x_b = sum_{c >= b} X_cb^T y_c, wave g takes columns g + 16u of
+//           panel b, its lanes run down the rows.
+// LDS: y (64 nb x SV_RC), partial sums (16 x 64 x SV_RC).
+__global__ void __launch_bounds__(SU_T) k_solve_diag(TLR t, int k, const double *Xinv, const double *Pin,
+                                                     double *Pout, long ldp, int nrhs, int trans)
 {
     extern __shared__ double sm[];
-    const int NB = t.NB, nbk = (NB + SV_BK - 1) / SV_BK, NBp = nbk * SV_BK;
-    double *x = sm;                     // x[l * SV_RC + q]
-    double *Ws = x + (size_t)NBp * SV_RC;  // Ws[i * SV_WL + c] = W(i, c)
-    double *pt = Ws + SV_BK * SV_WL;    // pt[(g * 64 + i) * SV_RC + q]
-    const int tid = threadIdx.x, i = tid & 63, g = tid >> 6, lane = i;
-    const double *L = band_tile(t, k, k);
-    double *Pk = P + (size_t)k * NB;
+    const int NB = t.NB, nb = (NB + SV_BK - 1) / SV_BK, NBp = nb * SV_BK;
+    double *y = sm;                            // y[l * SV_RC + q]
+    double *pt = y + (size_t)NBp * SV_RC;      // pt[(g * 64 + i) * SV_RC + q]
+    const int tid = threadIdx.x, i = tid & 63, g = tid >> 6, b = blockIdx.x;
+    const double *X0 = Xinv + (size_t)k * inv_tile(nb);
+    const double *Pk = Pin + (size_t)k * NB;
+    double *Po = Pout + (size_t)k * NB;
     for (int r0 = 0; r0 < nrhs; r0 += SV_RC) {
         const int nq = min(SV_RC, nrhs - r0);
-        for (int e = tid; e < NBp * SV_RC; e += 256) {
+        for (int e = tid; e < NBp * SV_RC; e += SU_T) {
             const int q = e / NBp, l = e - q * NBp;
-            x[l * SV_RC + q] = (l < NB && q < nq) ? Pk[l + (size_t)(r0 + q) * ldp] : 0.0;
+            y[l * SV_RC + q] = (l < NB && q < nq) ? Pk[l + (size_t)(r0 + q) * ldp] : 0.0;
         }
         __syncthreads();
-        for (int s = 0; s < nbk; s++) {
-            const int b = trans ? nbk - 1 - s : s;
-            const int c0 = SV_BK * b;
+        if (!trans) {
             double acc[SV_RC];
 #pragma unroll
             for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
-            if (!trans) {
-                // rows of block b times the solved part x(0 : c0): thread (i, g)
-                // takes the columns l = g, g + 4, ... (coalesced down a column)
-                const int row = c0 + i;
-                if (row < NB)
-                    for (int l = g; l < c0; l += 4) {
-                        const double lv = L[row + (size_t)l * NB];
+#pragma unroll 2
+            for (int c = 0; c <= b; c++) {
+                const size_t ld = (size_t)SV_BK * (nb - c);
+                const double *Bp = X0 + inv_poff(nb, c) + SV_BK * (b - c) + i;
+                double v[SV_BK / SU_G];
 #pragma unroll
-                        for (int q = 0; q < SV_RC; q++) acc[q] = fma(lv, x[l * SV_RC + q], acc[q]);
-                    }
+                for (int u = 0; u < SV_BK / SU_G; u++) v[u] = Bp[(size_t)(g + SU_G * u) * ld];
 #pragma unroll
-                for (int q = 0; q < SV_RC; q++) pt[(g * SV_BK + i) * SV_RC + q] = acc[q];
-            } else {
-                // columns of block b times the solved part x(c0 + 64 : NB):
-                // wave g takes the columns c0 + g, c0 + g + 4, ...; its lanes
-                // run down the column (coalesced), then a wave sum
-                for (int cc = g; cc < SV_BK; cc += 4) {
-                    const int col = c0 + cc;
-                    double a2[SV_RC];
+                for (int u = 0; u < SV_BK / SU_G; u++)
 #pragma unroll
-                    for (int q = 0; q < SV_RC; q++) a2[q] = 0.0;
-                    if (col < NB)
-                        for (int l = c0 + SV_BK + lane; l < NB; l += 64) {
-                            const double lv = L[l + (size_t)col * NB];
-#pragma unroll
-                            for (int q = 0; q < SV_RC; q++) a2[q] = fma(lv, x[l * SV_RC + q], a2[q]);
-                        }
-#pragma unroll
-                    for (int q = 0; q < SV_RC; q++) {
-                        const double v = wsum(a2[q]);
-                        if (lane == 0) pt[(0 * SV_BK + cc) * SV_RC + q] = v;
-                    }
-                }
+                    for (int q = 0; q < SV_RC; q++)
+                        acc[q] = fma(v[u], y[(SV_BK * c + g + SU_G * u) * SV_RC + q], acc[q]);
             }
-            const double *Wb = Winv + ((size_t)k * nbk + b) * SV_BK * SV_BK;
-            for (int e = tid; e < SV_BK * SV_BK; e += 256) Ws[(e >> 6) * SV_WL + (e & 63)] = Wb[e];
-            __syncthreads();
-            // y = x_b - (partial sums), in pt slot 0
-            if (g == 0) {
+#pragma unroll
+            for (int q = 0; q < SV_RC; q++) pt[(g * SV_BK + i) * SV_RC + q] = acc[q];
+        } else {
+            const size_t ld = (size_t)SV_BK * (nb - b);
+            const double *Pb = X0 + inv_poff(nb, b) + i;
+            double acc[SV_BK / SU_G][SV_RC];
+#pragma unroll
+            for (int u = 0; u < SV_BK / SU_G; u++)
+#pragma unroll
+                for (int q = 0; q < SV_RC; q++) acc[u][q] = 0.0;
+#pragma unroll 2
+            for (int c = b; c < nb; c++) {
+                double v[SV_BK / SU_G];
+#pragma unroll
+                for (int u = 0; u < SV_BK / SU_G; u++) v[u] = Pb[SV_BK * (c - b) + (size_t)(g + SU_G * u) * ld];
+#pragma unroll
+                for (int u = 0; u < SV_BK / SU_G; u++)
+#pragma unroll
+                    for (int q = 0; q < SV_RC; q++) acc[u][q] = fma(v[u], y[(SV_BK * c + i) * SV_RC + q], acc[u][q]);
+            }
+#pragma unroll
+            for (int u = 0; u < SV_BK / SU_G; u++)
 #pragma unroll
                 for (int q = 0; q < SV_RC; q++) {
-                    double ssum = pt[i * SV_RC + q];
-                    if (!trans)
-                        ssum = (ssum + pt[(SV_BK + i) * SV_RC + q]) +
-                               (pt[(2 * SV_BK + i) * SV_RC + q] + pt[(3 * SV_BK + i) * SV_RC + q]);
-                    pt[i * SV_RC + q] = x[(c0 + i) * SV_RC + q] - ssum;
+                    const double s = wsum(acc[u][q]);
+                    if (i == 0) pt[(g + SU_G * u) * SV_RC + q] = s;
                 }
-            }
-            __syncthreads();
-            // x_b = W y (forward) or W^T y (backward): thread (i, q = g)
-            {
-                const int q = g;
-                double v = 0.0;
-                if (!trans)
-                    for (int l = 0; l <= i; l++) v = fma(Ws[i * SV_WL + l], pt[l * SV_RC + q], v);
-                else
-                    for (int l = i; l < SV_BK; l++) v = fma(Ws[l * SV_WL + i], pt[l * SV_RC + q], v);
-                x[(c0 + i) * SV_RC + q] = v;
-            }
-            __syncthreads();
         }
-        for (int e = tid; e < NB * SV_RC; e += 256) {
-            const int q = e / NB, l = e - q * NB;
-            if (q < nq) Pk[l + (size_t)(r0 + q) * ldp] = x[l * SV_RC + q];
+        __syncthreads();
+        if (g == 0) {
+            const int row = SV_BK * b + i;
+#pragma unroll
+            for (int q = 0; q < SV_RC; q++) {
+                double s = pt[i * SV_RC + q];
+                if (!trans)
+                    for (int gg = 1; gg < SU_G; gg++) s += pt[(gg * SV_BK + i) * SV_RC + q];
+                if (row < NB && q < nq) Po[row + (size_t)(r0 + q) * ldp] = s;
+            }
         }
         __syncthreads();
     }
 }
 
-// The low-rank part of a solve update, [U | V] held as TE (widened on load,
-// FP64 arithmetic): w = F^T X_k with F = V (forward) or U (backward), a wave
-// per column; then P_o -= G w with G = U (forward) or V (backward).
+// w = F^T x (F = NB x r held as TE, x in LDS): wave g takes columns g + 16u,
+// its lanes run down the rows; w[c * SV_RC + q] in LDS
 template <typename TE>
-__device__ __forceinline__ void solve_lr(const TE *T, int NB, int r, int trans, const double *xk, double *w,
-                                         double *Po, long ldp, int nq)
+__device__ __forceinline__ void su_lr_w(const TE *F, int NB, int r, const double *x, double *w)
 {
-    const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
-    const TE *U = T, *V = T + (size_t)NB * r;
-    const TE *F = trans ? U : V, *Gm = trans ? V : U;
-    for (int c = g; c < r; c += 4) {
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    for (int c = g; c < r; c += SU_G) {
+        const TE *Fc = F + (size_t)c * NB;
         double acc[SV_RC];
 #pragma unroll
         for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
+#pragma unroll 4
         for (int i = lane; i < NB; i += 64) {
-            const double fv = (double)F[i + (size_t)c * NB];
+            const double fv = (double)Fc[i];
 #pragma unroll
-            for (int q = 0; q < SV_RC; q++) acc[q] = fma(fv, xk[i * SV_RC + q], acc[q]);
+            for (int q = 0; q < SV_RC; q++) acc[q] = fma(fv, x[i * SV_RC + q], acc[q]);
         }
 #pragma unroll
         for (int q = 0; q < SV_RC; q++) {
@@ -173,81 +209,126 @@ __device__ __forceinline__ void solve_lr(const TE *T, int NB, int r, int trans, 
             if (lane == 0) w[c * SV_RC + q] = v;
         }
     }
-    __syncthreads();
-    for (int i = tid; i < NB; i += 256) {
-        double acc[SV_RC];
+}
+
+// acc += G(row, :) w for one row per thread (i, g): columns g + 16u
+template <typename TE>
+__device__ __forceinline__ void su_lr_rows(const TE *G, int NB, int r, int row, const double *w, double (&acc)[SV_RC])
+{
+    const int g = threadIdx.x >> 6;
+#pragma unroll 4
+    for (int c = g; c < r; c += SU_G) {
+        const double gv = (double)G[row + (size_t)c * NB];
 #pragma unroll
-        for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
-        for (int c = 0; c < r; c++) {
-            const double gv = (double)Gm[i + (size_t)c * NB];
-#pragma unroll
-            for (int q = 0; q < SV_RC; q++) acc[q] = fma(gv, w[c * SV_RC + q], acc[q]);
-        }
-        for (int q = 0; q < nq; q++) Po[i + (size_t)q * ldp] -= acc[q];
+        for (int q = 0; q < SV_RC; q++) acc[q] = fma(gv, w[c * SV_RC + q], acc[q]);
     }
 }
 
-// The update of one block of right-hand sides by the solved block k:
-// forward: P_m -= L_mk X_k for m = k + 1 + blockIdx.x; backward:
-// P_j -= L_kj^T X_k for j = blockIdx.x.  LDS: X_k (NB x SV_RC), w (cap x SV_RC).
-__global__ void __launch_bounds__(256) k_solve_update(TLR t, int k, double *P, long ldp, int nrhs, int trans)
+// The update by the solved block k of 64 rows (row block rb) of one block
+// of the right-hand sides in Pupd, x_k read from Psol:
+// forward:  P_m -= L_mk x_k for m = k + 1 + tile;
+// backward: P_j -= L_kj^T x_k for j = tile;
+// with L x = U (V^T x) for low-rank tiles (F^T x computed by every row
+// block of the tile; the row blocks of a tile share blockIdx mod 8, so they
+// run on one XCD and F comes from that XCD's L2).  Grid: nb x T8 workgroups,
+// tile = blockIdx % T8, rb = blockIdx / T8, T8 = ntile rounded up to 8.
+// LDS: x_k (NB x SV_RC), w (cap x SV_RC), partial sums (16 x 64 x SV_RC).
+__global__ void __launch_bounds__(SU_T) k_solve_update(TLR t, int k, const double *Psol, double *Pupd, long ldp,
+                                                       int nrhs, int trans, int ntile, int T8)
 {
     extern __shared__ double sm[];
     const int NB = t.NB;
-    double *xk = sm;                        // xk[l * SV_RC + q]
-    double *w = xk + (size_t)NB * SV_RC;    // w[c * SV_RC + q]
-    const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
+    const int tile = blockIdx.x % T8, rb = blockIdx.x / T8;
+    if (tile >= ntile) return;
+    double *xk = sm;                            // xk[l * SV_RC + q]
+    double *w = xk + (size_t)NB * SV_RC;        // w[c * SV_RC + q]
+    double *pt = w + (size_t)t.cap * SV_RC;     // pt[(g * 64 + i) * SV_RC + q]
+    const int tid = threadIdx.x, i = tid & 63, g = tid >> 6;
     int m, n, o;
-    if (!trans) { m = k + 1 + blockIdx.x; n = k; o = m; }
-    else { m = k; n = blockIdx.x; o = n; }
+    if (!trans) { m = k + 1 + tile; n = k; o = m; }
+    else { m = k; n = tile; o = n; }
     const bool dense = (m - n < t.band_size);
     const int r = dense ? 0 : *rank_at(t, m, n);
     if (!dense && r <= 0) return;
     const double *T = dense ? band_tile(t, m, n) : lr_tile(t, m, n);
-    const double *Pk = P + (size_t)k * NB;
-    double *Po = P + (size_t)o * NB;
+    const int p = dense ? LR_F64 : lr_prec(t, m, n);
+    const double *Pk = Psol + (size_t)k * NB;
+    double *Po = Pupd + (size_t)o * NB;
+    const int row = SV_BK * rb + i;
     for (int r0 = 0; r0 < nrhs; r0 += SV_RC) {
         const int nq = min(SV_RC, nrhs - r0);
-        for (int e = tid; e < NB * SV_RC; e += 256) {
+        for (int e = tid; e < NB * SV_RC; e += SU_T) {
             const int q = e / NB, l = e - q * NB;
             xk[l * SV_RC + q] = (q < nq) ? Pk[l + (size_t)(r0 + q) * ldp] : 0.0;
         }
         __syncthreads();
-        if (dense && !trans) {
-            // P_m(i) -= sum_l L_mk(i, l) X_k(l): a thread per row
-            for (int i = tid; i < NB; i += 256) {
-                double acc[SV_RC];
+        double acc[SV_RC];
 #pragma unroll
-                for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
-                for (int l = 0; l < NB; l++) {
-                    const double lv = T[i + (size_t)l * NB];
+        for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
+        bool waves = false;  // sums left by lane 0 of each wave in pt slot 0
+        if (dense && !trans) {
+            // P_m(row) -= sum_l L_mk(row, l) x_k(l): thread (i, g), l = g + 16u
+            if (row < NB) {
+                const double *Tr = T + row;
+#pragma unroll 8
+                for (int l = g; l < NB; l += SU_G) {
+                    const double lv = Tr[(size_t)l * NB];
 #pragma unroll
                     for (int q = 0; q < SV_RC; q++) acc[q] = fma(lv, xk[l * SV_RC + q], acc[q]);
                 }
-                for (int q = 0; q < nq; q++) Po[i + (size_t)(r0 + q) * ldp] -= acc[q];
             }
         } else if (dense) {
-            // P_j(l) -= sum_i L_kj(i, l) X_k(i): a wave per column l
-            for (int l = g; l < NB; l += 4) {
-                double acc[SV_RC];
+            // P_j(l) -= sum_i L_kj(i, l) x_k(i) for the 64 columns l of the row
+            // block: wave g takes l = 64 rb + g + 16u, its lanes run down i
+            waves = true;
+            for (int u = 0; u < SV_BK / SU_G; u++) {
+                const int l = SV_BK * rb + g + SU_G * u;
+                double a2[SV_RC];
 #pragma unroll
-                for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
-                for (int i = lane; i < NB; i += 64) {
-                    const double lv = T[i + (size_t)l * NB];
+                for (int q = 0; q < SV_RC; q++) a2[q] = 0.0;
+                if (l < NB) {
+                    const double *Tl = T + (size_t)l * NB;
+#pragma unroll 4
+                    for (int ii = i; ii < NB; ii += 64) {
+                        const double lv = Tl[ii];
 #pragma unroll
-                    for (int q = 0; q < SV_RC; q++) acc[q] = fma(lv, xk[i * SV_RC + q], acc[q]);
+                        for (int q = 0; q < SV_RC; q++) a2[q] = fma(lv, xk[ii * SV_RC + q], a2[q]);
+                    }
                 }
 #pragma unroll
                 for (int q = 0; q < SV_RC; q++) {
-                    const double v = wsum(acc[q]);
-                    if (lane == 0 && q < nq) Po[l + (size_t)(r0 + q) * ldp] -= v;
+                    const double s = wsum(a2[q]);
+                    if (i == 0) pt[(g + SU_G * u) * SV_RC + q] = s;
                 }
             }
         } else {
-            const int p = lr_prec(t, m, n);
-            if (p == LR_F64) solve_lr(T, NB, r, trans, xk, w, Po + (size_t)r0 * ldp, ldp, nq);
-            else if (p == LR_F32) solve_lr((const float *)T, NB, r, trans, xk, w, Po + (size_t)r0 * ldp, ldp, nq);
-            else solve_lr((const lr_bf16 *)T, NB, r, trans, xk, w, Po + (size_t)r0 * ldp, ldp, nq);
+            // low rank: w = F^T x_k (F = V forward, U backward), then the rows
+            // of G w (G = U forward, V backward)
+            const size_t off = (size_t)NB * r;
+            const int fo = trans ? 0 : 1;
+            if (p == LR_F64) su_lr_w(T + fo * off, NB, r, xk, w);
+            else if (p == LR_F32) su_lr_w((const float *)T + fo * off, NB, r, xk, w);
+            else su_lr_w((const lr_bf16 *)T + fo * off, NB, r, xk, w);
+            __syncthreads();
+            if (row < NB) {
+                if (p == LR_F64) su_lr_rows(T + (1 - fo) * off, NB, r, row, w, acc);
+                else if (p == LR_F32) su_lr_rows((const float *)T + (1 - fo) * off, NB, r, row, w, acc);
+                else su_lr_rows((const lr_bf16 *)T + (1 - fo) * off, NB, r, row, w, acc);
+            }
+        }
+        if (!waves) {
+#pragma unroll
+            for (int q = 0; q < SV_RC; q++) pt[(g * SV_BK + i) * SV_RC + q] = acc[q];
+        }
+        __syncthreads();
+        if (g == 0 && row < NB) {
+#pragma unroll
+            for (int q = 0; q < SV_RC; q++) {
+                double s = pt[i * SV_RC + q];
+                if (!waves)
+                    for (int gg = 1; gg < SU_G; gg++) s += pt[(gg * SV_BK + i) * SV_RC + q];
+                if (q < nq) Po[row + (size_t)(r0 + q) * ldp] -= s;
+            }
         }
         __syncthreads();
     }
@@ -303,13 +384,14 @@ __global__ void __launch_bounds__(256) k_logdet_tiles(TLR t, double *out)
 // ---------------------------------------------------------------------------
 // host side
 
-// doubles of workspace a solve needs: block inverses, padded right-hand
-// sides, reduction parts, three scalars
+// doubles of workspace a solve needs: the panels of every L_kk^-1, two
+// padded right-hand-side blocks (a pass reads one and writes the other),
+// reduction parts, three scalars
 size_t solve_ws_doubles(const TLR &t, int nrhs)
 {
-    const size_t nbk = (size_t)(t.NB + SV_BK - 1) / SV_BK;
+    const int nbk = (t.NB + SV_BK - 1) / SV_BK;
     const size_t Np = (size_t)t.NT * t.NB;
-    return nbk * t.NT * SV_BK * SV_BK + Np * (size_t)nrhs + (Np + 255) / 256 + 8;
+    return (size_t)t.NT * inv_tile(nbk) + 2 * Np * (size_t)nrhs + (Np + 255) / 256 + 8;
 }
 
 size_t hicma_solve_bytes(int N, int NB)
@@ -324,9 +406,12 @@ size_t hicma_solve_bytes(int N, int NB)
 static size_t diag_lds(const TLR &t)
 {
     const size_t nbk = (size_t)(t.NB + SV_BK - 1) / SV_BK;
-    return sizeof(double) * (nbk * SV_BK * SV_RC + SV_BK * SV_WL + 4 * SV_BK * SV_RC);
+    return sizeof(double) * (nbk * SV_BK * SV_RC + (size_t)SU_G * SV_BK * SV_RC);
 }
-static size_t update_lds(const TLR &t) { return sizeof(double) * ((size_t)t.NB + (size_t)t.cap) * SV_RC; }
+static size_t update_lds(const TLR &t)
+{
+    return sizeof(double) * (((size_t)t.NB + (size_t)t.cap) * SV_RC + (size_t)SU_G * SV_BK * SV_RC);
+}
 
 // dirs: 1 forward, 2 backward, 3 both.  B: device, N x nrhs, ld ldb.
 // quad (if not null): ||L^-1 B(:, 0)||^2 after the forward pass.  ws: device
@@ -338,37 +423,46 @@ int solve_launch(hicma_ctx *ctx, hicma_tlr_desc *A, int dirs, int nrhs, double *
     const int NT = t.NT, NB = t.NB, nbk = (NB + SV_BK - 1) / SV_BK;
     const long Np = (long)NT * NB;
     hipStream_t st = ctx->stream;
-    double *Winv = ws;
-    double *P = Winv + (size_t)nbk * NT * SV_BK * SV_BK;
-    double *parts = P + (size_t)Np * nrhs;
+    double *Xinv = ws;
+    double *P0 = Xinv + (size_t)NT * inv_tile(nbk);
+    double *P1 = P0 + (size_t)Np * nrhs;
+    double *parts = P1 + (size_t)Np * nrhs;
     const size_t ld1 = diag_lds(t), ld2 = update_lds(t);
     HIPCHK(hipFuncSetAttribute((const void *)k_solve_diag, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ld1));
     HIPCHK(hipFuncSetAttribute((const void *)k_solve_update, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ld2));
-    hipLaunchKernelGGL(k_tri_inv, dim3((unsigned)(NT * nbk)), dim3(64), 0, st, t, Winv);
+    hipLaunchKernelGGL(k_tri_inv, dim3((unsigned)(NT * nbk)), dim3(64), 0, st, t, Xinv);
+    if (nbk > 1) hipLaunchKernelGGL(k_tile_inv, dim3((unsigned)(NT * nbk)), dim3(256), 0, st, t, Xinv);
     const long ne = Np * nrhs;
-    hipLaunchKernelGGL(k_pad_copy, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, (const double *)B, ldb, P, Np,
+    hipLaunchKernelGGL(k_pad_copy, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, (const double *)B, ldb, P0, Np,
                        t.N, (int)Np, nrhs);
-    if (dirs & 1) {
-        for (int k = 0; k < NT; k++) {
-            hipLaunchKernelGGL(k_solve_diag, dim3(1), dim3(256), ld1, st, t, k, (const double *)Winv, P, Np, nrhs, 0);
-            if (k + 1 < NT)
-                hipLaunchKernelGGL(k_solve_update, dim3(NT - 1 - k), dim3(256), ld2, st, t, k, P, Np, nrhs, 0);
+    // each pass reads the right-hand sides from src, leaves its solution in dst
+    double *src = P0, *dst = P1;
+    auto pass = [&](int trans) {
+        for (int s = 0; s < NT; s++) {
+            const int k = trans ? NT - 1 - s : s;
+            hipLaunchKernelGGL(k_solve_diag, dim3((unsigned)nbk), dim3(SU_T), ld1, st, t, k, (const double *)Xinv,
+                               (const double *)src, dst, Np, nrhs, trans);
+            const int ntile = trans ? k : NT - 1 - k;
+            if (ntile > 0) {
+                const int T8 = (ntile + 7) / 8 * 8;
+                hipLaunchKernelGGL(k_solve_update, dim3((unsigned)(T8 * nbk)), dim3(SU_T), ld2, st, t, k,
+                                   (const double *)dst, src, Np, nrhs, trans, ntile, T8);
+            }
         }
+        std::swap(src, dst);
+    };
+    if (dirs & 1) {
+        pass(0);
         if (d_quad) {
             const int nb = (int)((t.N + 255) / 256);
-            hipLaunchKernelGGL(k_sumsq_parts, dim3(nb), dim3(256), 0, st, (const double *)P, t.N, parts);
+            hipLaunchKernelGGL(k_sumsq_parts, dim3(nb), dim3(256), 0, st, (const double *)src, t.N, parts);
             hipLaunchKernelGGL(k_sum_seq, dim3(1), dim3(256), 0, st, (const double *)parts, nb, d_quad);
         }
     }
-    if (dirs & 2) {
-        for (int k = NT - 1; k >= 0; k--) {
-            hipLaunchKernelGGL(k_solve_diag, dim3(1), dim3(256), ld1, st, t, k, (const double *)Winv, P, Np, nrhs, 1);
-            if (k > 0) hipLaunchKernelGGL(k_solve_update, dim3(k), dim3(256), ld2, st, t, k, P, Np, nrhs, 1);
-        }
-    }
+    if (dirs & 2) pass(1);
     if (d_logdet) hipLaunchKernelGGL(k_logdet_tiles, dim3(1), dim3(256), 0, st, t, d_logdet);
     hipLaunchKernelGGL(k_pad_copy, dim3((unsigned)(((long)t.N * nrhs + 255) / 256)), dim3(256), 0, st,
-                       (const double *)P, Np, B, ldb, (int)Np, t.N, nrhs);
+                       (const double *)src, Np, B, ldb, (int)Np, t.N, nrhs);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -38,8 +38,8 @@ def test_noalloc_matches_allocating_entry_point():
             assert np.isfinite(b_ll) and b_q > 0
             g.close()
         # a problem whose workspace exceeds the 8 Mi-double scratch is refused
-        # before anything runs: N = 200,000, NB = 2,000 needs 100 * 32 * 4096
-        # block-inverse doubles alone
+        # before anything runs: N = 200,000, NB = 2,000 needs 100 * 528 * 4096
+        # doubles for the panels of L_kk^-1 alone
         big = H.TLRMatrix(ctx, 200000, 2000, 10, H.SQEXP_2D, 1.0, 0.1, 1e-2, 0)
         X = H.DeviceArray(ctx, np.zeros((200000, 1)))
         ld = C.c_double()
