@@ -22,6 +22,7 @@
 #include "wave.hpp"
 
 #include <math.h>
+#include <algorithm>
 #include <utility>
 
 #define SV_RC 4                // right-hand sides per pass
@@ -211,35 +212,48 @@ __device__ __forceinline__ void su_lr_w(const TE *F, int NB, int r, const double
     }
 }
 
-// acc += G(row, :) w for one row per thread (i, g): columns g + 16u
+// P_o(row) -= G(row, :) w for every row, one row per thread (G = NB x r
+// held as TE)
 template <typename TE>
-__device__ __forceinline__ void su_lr_rows(const TE *G, int NB, int r, int row, const double *w, double (&acc)[SV_RC])
+__device__ __forceinline__ void su_lr_rows(const TE *G, int NB, int r, const double *w, double *Po, long ldp, int nq)
 {
-    const int g = threadIdx.x >> 6;
-#pragma unroll 4
-    for (int c = g; c < r; c += SU_G) {
-        const double gv = (double)G[row + (size_t)c * NB];
+    for (int row = threadIdx.x; row < NB; row += SU_T) {
+        double acc[SV_RC];
 #pragma unroll
-        for (int q = 0; q < SV_RC; q++) acc[q] = fma(gv, w[c * SV_RC + q], acc[q]);
+        for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
+#pragma unroll 4
+        for (int c = 0; c < r; c++) {
+            const double gv = (double)G[row + (size_t)c * NB];
+#pragma unroll
+            for (int q = 0; q < SV_RC; q++) acc[q] = fma(gv, w[c * SV_RC + q], acc[q]);
+        }
+        for (int q = 0; q < nq; q++) Po[row + (size_t)q * ldp] -= acc[q];
     }
 }
 
-// The update by the solved block k of 64 rows (row block rb) of one block
-// of the right-hand sides in Pupd, x_k read from Psol:
-// forward:  P_m -= L_mk x_k for m = k + 1 + tile;
-// backward: P_j -= L_kj^T x_k for j = tile;
-// with L x = U (V^T x) for low-rank tiles (F^T x computed by every row
-// block of the tile; the row blocks of a tile share blockIdx mod 8, so they
-// run on one XCD and F comes from that XCD's L2).  Grid: nb x T8 workgroups,
-// tile = blockIdx % T8, rb = blockIdx / T8, T8 = ntile rounded up to 8.
+// The update by the solved block k of one block of the right-hand sides in
+// Pupd, x_k read from Psol:
+// forward:  P_m -= L_mk x_k for m = k + 1 + tile, tile < ntile;
+// backward: P_j -= L_kj^T x_k for j = tile, tile < ntile = k;
+// the ndense tiles inside the band (the first ones forward, the last ones
+// backward) take one workgroup per 64-row block (rb), each low-rank tile one
+// workgroup: w = F^T x_k (F = V forward, U backward), then the rows of G w
+// (G = U forward, V backward).  Grid: ndense nb + (ntile - ndense).
 // LDS: x_k (NB x SV_RC), w (cap x SV_RC), partial sums (16 x 64 x SV_RC).
 __global__ void __launch_bounds__(SU_T) k_solve_update(TLR t, int k, const double *Psol, double *Pupd, long ldp,
-                                                       int nrhs, int trans, int ntile, int T8)
+                                                       int nrhs, int trans, int ndense)
 {
     extern __shared__ double sm[];
-    const int NB = t.NB;
-    const int tile = blockIdx.x % T8, rb = blockIdx.x / T8;
-    if (tile >= ntile) return;
+    const int NB = t.NB, nb = (NB + SV_BK - 1) / SV_BK;
+    int tile, rb = -1;
+    if ((int)blockIdx.x < ndense * nb) {
+        const int d = blockIdx.x / nb;
+        rb = blockIdx.x - d * nb;
+        tile = trans ? k - ndense + d : d;
+    } else {
+        const int e = blockIdx.x - ndense * nb;
+        tile = trans ? e : ndense + e;
+    }
     double *xk = sm;                            // xk[l * SV_RC + q]
     double *w = xk + (size_t)NB * SV_RC;        // w[c * SV_RC + q]
     double *pt = w + (size_t)t.cap * SV_RC;     // pt[(g * 64 + i) * SV_RC + q]
@@ -247,11 +261,10 @@ __global__ void __launch_bounds__(SU_T) k_solve_update(TLR t, int k, const doubl
     int m, n, o;
     if (!trans) { m = k + 1 + tile; n = k; o = m; }
     else { m = k; n = tile; o = n; }
-    const bool dense = (m - n < t.band_size);
-    const int r = dense ? 0 : *rank_at(t, m, n);
-    if (!dense && r <= 0) return;
-    const double *T = dense ? band_tile(t, m, n) : lr_tile(t, m, n);
-    const int p = dense ? LR_F64 : lr_prec(t, m, n);
+    const int r = (rb >= 0) ? 0 : *rank_at(t, m, n);
+    if (rb < 0 && r <= 0) return;
+    const double *T = (rb >= 0) ? band_tile(t, m, n) : lr_tile(t, m, n);
+    const int p = (rb >= 0) ? LR_F64 : lr_prec(t, m, n);
     const double *Pk = Psol + (size_t)k * NB;
     double *Po = Pupd + (size_t)o * NB;
     const int row = SV_BK * rb + i;
@@ -262,12 +275,25 @@ __global__ void __launch_bounds__(SU_T) k_solve_update(TLR t, int k, const doubl
             xk[l * SV_RC + q] = (q < nq) ? Pk[l + (size_t)(r0 + q) * ldp] : 0.0;
         }
         __syncthreads();
-        double acc[SV_RC];
-#pragma unroll
-        for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
-        bool waves = false;  // sums left by lane 0 of each wave in pt slot 0
-        if (dense && !trans) {
+        if (rb < 0) {
+            const size_t off = (size_t)NB * r;
+            const int fo = trans ? 0 : 1;
+            double *Pq = Po + (size_t)r0 * ldp;
+            if (p == LR_F64) su_lr_w(T + fo * off, NB, r, xk, w);
+            else if (p == LR_F32) su_lr_w((const float *)T + fo * off, NB, r, xk, w);
+            else su_lr_w((const lr_bf16 *)T + fo * off, NB, r, xk, w);
+            __syncthreads();
+            if (p == LR_F64) su_lr_rows(T + (1 - fo) * off, NB, r, w, Pq, ldp, nq);
+            else if (p == LR_F32) su_lr_rows((const float *)T + (1 - fo) * off, NB, r, w, Pq, ldp, nq);
+            else su_lr_rows((const lr_bf16 *)T + (1 - fo) * off, NB, r, w, Pq, ldp, nq);
+            __syncthreads();
+            continue;
+        }
+        if (!trans) {
             // P_m(row) -= sum_l L_mk(row, l) x_k(l): thread (i, g), l = g + 16u
+            double acc[SV_RC];
+#pragma unroll
+            for (int q = 0; q < SV_RC; q++) acc[q] = 0.0;
             if (row < NB) {
                 const double *Tr = T + row;
 #pragma unroll 8
@@ -277,10 +303,11 @@ __global__ void __launch_bounds__(SU_T) k_solve_update(TLR t, int k, const doubl
                     for (int q = 0; q < SV_RC; q++) acc[q] = fma(lv, xk[l * SV_RC + q], acc[q]);
                 }
             }
-        } else if (dense) {
+#pragma unroll
+            for (int q = 0; q < SV_RC; q++) pt[(g * SV_BK + i) * SV_RC + q] = acc[q];
+        } else {
             // P_j(l) -= sum_i L_kj(i, l) x_k(i) for the 64 columns l of the row
             // block: wave g takes l = 64 rb + g + 16u, its lanes run down i
-            waves = true;
             for (int u = 0; u < SV_BK / SU_G; u++) {
                 const int l = SV_BK * rb + g + SU_G * u;
                 double a2[SV_RC];
@@ -301,31 +328,13 @@ __global__ void __launch_bounds__(SU_T) k_solve_update(TLR t, int k, const doubl
                     if (i == 0) pt[(g + SU_G * u) * SV_RC + q] = s;
                 }
             }
-        } else {
-            // low rank: w = F^T x_k (F = V forward, U backward), then the rows
-            // of G w (G = U forward, V backward)
-            const size_t off = (size_t)NB * r;
-            const int fo = trans ? 0 : 1;
-            if (p == LR_F64) su_lr_w(T + fo * off, NB, r, xk, w);
-            else if (p == LR_F32) su_lr_w((const float *)T + fo * off, NB, r, xk, w);
-            else su_lr_w((const lr_bf16 *)T + fo * off, NB, r, xk, w);
-            __syncthreads();
-            if (row < NB) {
-                if (p == LR_F64) su_lr_rows(T + (1 - fo) * off, NB, r, row, w, acc);
-                else if (p == LR_F32) su_lr_rows((const float *)T + (1 - fo) * off, NB, r, row, w, acc);
-                else su_lr_rows((const lr_bf16 *)T + (1 - fo) * off, NB, r, row, w, acc);
-            }
-        }
-        if (!waves) {
-#pragma unroll
-            for (int q = 0; q < SV_RC; q++) pt[(g * SV_BK + i) * SV_RC + q] = acc[q];
         }
         __syncthreads();
         if (g == 0 && row < NB) {
 #pragma unroll
             for (int q = 0; q < SV_RC; q++) {
                 double s = pt[i * SV_RC + q];
-                if (!waves)
+                if (!trans)
                     for (int gg = 1; gg < SU_G; gg++) s += pt[(gg * SV_BK + i) * SV_RC + q];
                 if (q < nq) Po[row + (size_t)(r0 + q) * ldp] -= s;
             }
@@ -366,32 +375,17 @@ __global__ void __launch_bounds__(256) k_sum_seq(const double *part, int n, doub
     if (threadIdx.x == 0) *out = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// 2 sum log L(i, i) over the true N, in the order of k_logdet (hicma_logdet)
-__global__ void __launch_bounds__(256) k_logdet_tiles(TLR t, double *out)
-{
-    __shared__ double red[4];
-    double part = 0.0;
-    for (int gi = threadIdx.x; gi < t.N; gi += 256) {
-        const int k = gi / t.NB, i = gi % t.NB;
-        part += log(band_tile(t, k, k)[i + (size_t)i * t.NB]);
-    }
-    part = wsum(part);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
-    __syncthreads();
-    if (threadIdx.x == 0) *out = 2.0 * (((red[0] + red[1]) + red[2]) + red[3]);
-}
-
 // ---------------------------------------------------------------------------
 // host side
 
 // doubles of workspace a solve needs: the panels of every L_kk^-1, two
 // padded right-hand-side blocks (a pass reads one and writes the other),
-// reduction parts, three scalars
+// reduction parts (of the sum of squares, then of the log-det), three scalars
 size_t solve_ws_doubles(const TLR &t, int nrhs)
 {
     const int nbk = (t.NB + SV_BK - 1) / SV_BK;
     const size_t Np = (size_t)t.NT * t.NB;
-    return (size_t)t.NT * inv_tile(nbk) + 2 * Np * (size_t)nrhs + (Np + 255) / 256 + 8;
+    return (size_t)t.NT * inv_tile(nbk) + 2 * Np * (size_t)nrhs + std::max((Np + 255) / 256, (size_t)t.NT) + 8;
 }
 
 size_t hicma_solve_bytes(int N, int NB)
@@ -443,11 +437,10 @@ int solve_launch(hicma_ctx *ctx, hicma_tlr_desc *A, int dirs, int nrhs, double *
             hipLaunchKernelGGL(k_solve_diag, dim3((unsigned)nbk), dim3(SU_T), ld1, st, t, k, (const double *)Xinv,
                                (const double *)src, dst, Np, nrhs, trans);
             const int ntile = trans ? k : NT - 1 - k;
-            if (ntile > 0) {
-                const int T8 = (ntile + 7) / 8 * 8;
-                hipLaunchKernelGGL(k_solve_update, dim3((unsigned)(T8 * nbk)), dim3(SU_T), ld2, st, t, k,
-                                   (const double *)dst, src, Np, nrhs, trans, ntile, T8);
-            }
+            const int ndense = std::min(t.band_size - 1, ntile);
+            if (ntile > 0)
+                hipLaunchKernelGGL(k_solve_update, dim3((unsigned)(ndense * nbk + ntile - ndense)), dim3(SU_T), ld2,
+                                   st, t, k, (const double *)dst, src, Np, nrhs, trans, ndense);
         }
         std::swap(src, dst);
     };
@@ -460,7 +453,7 @@ int solve_launch(hicma_ctx *ctx, hicma_tlr_desc *A, int dirs, int nrhs, double *
         }
     }
     if (dirs & 2) pass(1);
-    if (d_logdet) hipLaunchKernelGGL(k_logdet_tiles, dim3(1), dim3(256), 0, st, t, d_logdet);
+    if (d_logdet) logdet_launch(t, parts, d_logdet, st);
     hipLaunchKernelGGL(k_pad_copy, dim3((unsigned)(((long)t.N * nrhs + 255) / 256)), dim3(256), 0, st,
                        (const double *)src, Np, B, ldb, (int)Np, t.N, nrhs);
     HIPCHK(hipGetLastError());

@@ -1334,12 +1334,22 @@ extern "C" int hicma_set_tile(hicma_tlr_desc *A, int m, int n, const double *buf
     return 0;
 }
 
+// 2 sum log L(i, i) into *out (device), asynchronous on s, with NT doubles
+// of device workspace in parts; the log-likelihood uses the same kernels,
+// so both give the same bits
+void logdet_launch(const TLR &t, double *parts, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_logdet_parts, dim3(t.NT), dim3(LD_T), 0, s, t, parts);
+    hipLaunchKernelGGL(k_logdet_sum, dim3(1), dim3(64), 0, s, (const double *)parts, t.NT, out);
+}
+
 extern "C" int hicma_logdet(hicma_tlr_desc *A, double *logdet)
 {
     if (!A || !logdet) return HICMA_ERR_ARG;
     hicma_ctx *c = A->ctx;
     HIPCHK(hipSetDevice(c->device));
-    hipLaunchKernelGGL(k_logdet, dim3(1), dim3(BS), 0, c->stream, A->t, c->d_scalar);
+    RCHK(hicma_ensure_ws(c, sizeof(double) * (size_t)A->t.NT));
+    logdet_launch(A->t, c->ws, c->d_scalar, c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(logdet, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost));

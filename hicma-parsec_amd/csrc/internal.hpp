@@ -109,6 +109,7 @@ size_t hicma_work_bytes(hicma_ctx *ctx, int N, int NB, int cap, int band);
 int hicma_noalloc_setup(int device);
 // the solve's launch sequence and its helpers (hicma_solve.hip)
 size_t solve_ws_doubles(const TLR &t, int nrhs);
+void logdet_launch(const TLR &t, double *parts, double *out, hipStream_t s);
 int check_solve_args(hicma_ctx *ctx, hicma_tlr_desc *A, int nrhs, const double *B, long ldb);
 int solve_launch(hicma_ctx *ctx, hicma_tlr_desc *A, int dirs, int nrhs, double *B, long ldb, double *ws,
                  double *d_quad, double *d_logdet);

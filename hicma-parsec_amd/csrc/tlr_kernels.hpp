@@ -819,17 +819,34 @@ __global__ void __launch_bounds__(BS) k_lr_gemm(TLR t, int k, const int *tasks_s
 
 // ======================= utilities =======================
 
-// 2*sum(log(diag)) over the true N, fixed reduction order; one workgroup
-__global__ void __launch_bounds__(BS) k_logdet(TLR t, double *out)
+// 2*sum(log(diag)) over the true N in a fixed order: one workgroup per
+// diagonal tile sums its part (a wave sum per 64 rows, then the waves in
+// order), then k_logdet_sum adds the NT parts in order
+#define LD_T 1024
+__global__ void __launch_bounds__(LD_T) k_logdet_parts(TLR t, double *parts)
 {
-    __shared__ double red[NWAVE];
+    __shared__ double red[LD_T / 64];
+    const int k = blockIdx.x, rows = tdim(t, k);
+    const double *T = band_tile(t, k, k);
     double part = 0.0;
-    for (int g = threadIdx.x; g < t.N; g += BS) {
-        int k = g / t.NB, i = g % t.NB;
-        part += log(band_tile(t, k, k)[i + (size_t)i * t.NB]);
+    for (int i = threadIdx.x; i < rows; i += LD_T) part += log(T[i + (size_t)i * t.NB]);
+    part = wsum(part);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < LD_T / 64; w++) s += red[w];
+        parts[k] = s;
     }
-    double s = blk_sum(part, red);
-    if (threadIdx.x == 0) *out = 2.0 * s;
+}
+
+__global__ void k_logdet_sum(const double *parts, int n, double *out)
+{
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < n; k++) s += parts[k];
+        *out = 2.0 * s;
+    }
 }
 
 // rank history of the off-band tiles: phase 0 sets initial = min = max =

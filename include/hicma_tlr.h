@@ -275,7 +275,8 @@ int hicma_tlr_demote(hicma_tlr_desc *A, double thr32, double thr16, int *counts)
  * reference: for callers that cannot allocate (code under HIP graph capture,
  * serving loops).  Its workspace is a 64 MiB device array of the library
  * (8 Mi doubles: the lower 64 x 64 blocks of every L_kk^-1,
- * NT*4096*nb*(nb+1)/2 doubles with nb = (NB+63)/64, plus 2*NT*NB + NT*NB/256 + 9);
+ * NT*4096*nb*(nb+1)/2 doubles with nb = (NB+63)/64, plus 2*NT*NB,
+ * max(NT*NB/256 + 1, NT) and 8);
  * larger problems return HICMA_ERR_ARG.  Calls that
  * overlap on one GPU race on that array. */
 int hicma_tlr_loglik_noalloc(hicma_ctx *ctx, hicma_tlr_desc *A, double *z, double *logdet, double *quad,
