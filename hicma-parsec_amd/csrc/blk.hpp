@@ -420,26 +420,3 @@ __device__ void blk_sort_pos(int n, const double *sigma, int *pos)
     __syncthreads();
 }
 
-// X L^T = C in place for the rows [0, nr) of C (nr x n, ld ldc), L (n x n
-// lower, ld ldl).  Blocked by GT over columns.
-__device__ void blk_trsm_rltn(int nr, int n, const double *L, int ldl, double *C, int ldc, double *sm)
-{
-    const int tid = threadIdx.x;
-    for (int j0 = 0; j0 < n; j0 += GT) {
-        const int jb = (n - j0) < GT ? (n - j0) : GT;
-        if (j0 > 0)
-            blk_gemm<false, true>(nr, jb, j0, -1.0, C, ldc, L + j0, ldl, 1.0, C + (size_t)j0 * ldc, ldc, sm);
-        for (int j = 0; j < jb; j++) {
-            const int gj = j0 + j;
-            const double d = L[gj + (size_t)gj * ldl];
-            for (int r = tid; r < nr; r += BS) C[r + (size_t)gj * ldc] /= d;
-            __syncthreads();
-            const int cols = jb - j - 1;
-            for (int e = tid; e < cols * nr; e += BS) {
-                int r = e % nr, l = gj + 1 + e / nr;
-                C[r + (size_t)l * ldc] -= C[r + (size_t)gj * ldc] * L[l + (size_t)gj * ldl];
-            }
-            __syncthreads();
-        }
-    }
-}

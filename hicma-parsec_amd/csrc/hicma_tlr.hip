@@ -930,7 +930,8 @@ extern "C" int HiCMA_dpotrf_L(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma
         int nlr = NT - k - b;
         HIPCHK(cp(k, 2, st));
         if (nbnd > 0)
-            hipLaunchKernelGGL(k_trsm_band, dim3(ntg, nbnd), dim3(BS), 0, st, t, k, o);
+            hipLaunchKernelGGL(k_trsm_band, dim3(ntg, nbnd), dim3(BS), 0, st, t, k, (const double *)Winv,
+                               (const double *)Linv, o);
         if (nlr > 0)
             launch_trsm_lr(t, k, nlr, cap, Winv, Linv, o, st);
         HIPCHK(cp(k, 3, st));

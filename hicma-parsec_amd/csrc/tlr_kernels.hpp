@@ -531,19 +531,40 @@ __global__ void __launch_bounds__(BS) k_trsm_lr_inv(TLR t, int k, const double *
 
 // ======================= trsm =======================
 
-// band tiles m = k+1 .. k+band-1: C <- C L^{-T}; grid (ceil(NB/GT), nb)
-__global__ void __launch_bounds__(BS) k_trsm_band(TLR t, int k, OpLog o)
+// band tiles m = k+1 .. k+band-1: C <- C L^{-T} = C (L^{-1})^T on the matrix
+// cores with the block inverse of L_kk that k_potrf_dag leaves (Winv: the
+// diagonal blocks, Linv: the blocks below, 64 x 64 column major, Linv_bc at
+// Linv + (b nb + c) 4096).  One workgroup per 64-row stripe of a tile
+// (grid (ceil(NB/64), band-1)), in place: column block jb of the result,
+// X(:, jb) = sum_{ib <= jb} C(:, ib) Linv_{jb,ib}^T, reads column blocks
+// 0 .. jb of C only, so the blocks run from the last to the first.  For a
+// fixed jb the blocks Linv_{jb,0..jb-1} are contiguous, so the whole sum
+// over ib < jb is one operand with ld 64.
+__global__ void __launch_bounds__(BS) k_trsm_band(TLR t, int k, const double *Winv, const double *Linv, OpLog o)
 {
-    __shared__ double sm[BLK_GEMM_SMEM];
-    const int NB = t.NB;
-    int m = k + 1 + blockIdx.y;
-    int r0 = blockIdx.x * GT;
-    int nr = NB - r0 < GT ? NB - r0 : GT;
+    __shared__ double sm[MK_SMEM];
+    const int NB = t.NB, nb = (NB + PBK - 1) / PBK;
+    const int m = k + 1 + blockIdx.y, r0 = blockIdx.x * PBK;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        int mm = tdim(t, m);
+        const int mm = tdim(t, m);
         oplog(o, t.NT, opc('t', mm, mm, 1, 0), m, k, true, m - k == 1);
     }
-    blk_trsm_rltn(nr, NB, band_tile(t, k, k), NB, band_tile(t, m, k) + r0, NB, sm);
+    double *C = band_tile(t, m, k);
+    for (int jb = nb - 1; jb >= 0; jb--) {
+        const int kd = min(PBK, NB - PBK * jb);  // columns of block jb inside the tile
+        mk_d4 acc[2][2];
+        mk_zero(acc);
+        if (jb > 0) {
+            // B(kk, j) = L^-1(64 jb + j, kk) = Linv[(jb nb) 4096 + j + 64 kk] for kk < 64 jb
+            const MkOps<false, false> op{C, NB, Linv + (size_t)jb * nb * PBK * PBK, PBK};
+            mk_accum(op, NB, kd, r0, 0, 0, (long)PBK * jb, acc, sm);
+        }
+        const MkOps<false, false> od{C + (size_t)PBK * jb * NB, NB, Winv + (size_t)jb * PBK * PBK, PBK};
+        mk_accum(od, NB, kd, r0, 0, 0, kd, acc, sm);  // ends with a barrier: the block's reads are done
+        mk_store(acc, [&](int i, int j, double v) {
+            if (r0 + i < NB && j < kd) C[(r0 + i) + (size_t)(PBK * jb + j) * NB] = v;
+        });
+    }
 }
 
 // ======================= syrk =======================
