@@ -41,6 +41,9 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH_SIZE, WRITE_SIZE) summed over the group's kernels and scaled by the
 # counters' reading of a device copy of known size (x2.0 and x1.0).
 C2_TRAFFIC_BYTES_PER_GROUP = 6.399e9
+# HBM the serving sweep's admission may reserve (of 288 GB): the rest covers
+# the stream-ordered pool's cached blocks and allocation granularity
+SERVE_BUDGET_GB = 150.0
 
 
 def log(*a):
@@ -321,10 +324,10 @@ def main():
         pristine.close()
         ctx.close()
         rates = [float(x) for x in a.serve_rates.split(",") if x]
-        serving = {"slots": a.serve_slots, "hbm_budget_gb": 200.0,
+        serving = {"slots": a.serve_slots, "hbm_budget_gb": SERVE_BUDGET_GB,
                    "workload": ("Poisson arrivals; N = 512 k, k ~ U{8..128}; sigma2 ~ U[0.5, 2], beta ~ U[0.02, 0.2], "
                                 "nugget 1e-2; tile 512, eps 1e-8, maxrank 256; generate + factor + log-det + solve"),
-                   "sweep": serving_sweep(rates, a.serve_requests, a.serve_slots, 200.0)}
+                   "sweep": serving_sweep(rates, a.serve_requests, a.serve_slots, SERVE_BUDGET_GB)}
 
     out = {
         "metric": "TLR dpotrf seconds, 2D sqexp N=90k eps 1e-8",

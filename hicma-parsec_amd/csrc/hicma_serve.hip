@@ -34,7 +34,7 @@ size_t hicma_solve_bytes(int N, int NB);  // hicma_solve.hip
 // band choice of a request: the time model with this device's rates (dense
 // tiles about 10 TFLOP/s as the batched band kernels run, recompressions
 // about 3.5 TFLOP/s of reference op counts)
-#define SERVE_DENSE_TFLOPS 10.0
+#define SERVE_DENSE_TFLOPS 60.0
 #define SERVE_LR_TFLOPS 3.5
 
 namespace {
@@ -119,20 +119,30 @@ static SvRequest *admit(hicma_server *S)
     return nullptr;
 }
 
-// with S->mu not held: reserve `extra` more bytes for a running request.  It
-// waits until running requests have released enough; if every running
-// request is waiting here, this one goes over the budget (its allocations
-// may then fail, which ends it with HICMA_ERR_NOMEM).
-static void grow(hicma_server *S, SvRequest *q, size_t extra)
+// with S->mu not held: reserve the footprint of a dense band of `band` tile
+// diagonals for a running request admitted with a smaller one, and return the
+// band granted.  It waits until running requests have released enough; if
+// every running request is waiting here (none would release anything), it
+// takes the widest band whose footprint fits what is left of the budget
+// (band 1, the admitted footprint, always does), so the reservations never
+// exceed the budget.
+static int grow(hicma_server *S, SvRequest *q, int band)
 {
     std::unique_lock<std::mutex> lk(S->mu);
     S->growing++;
     S->cv_work.notify_all();
-    S->cv_work.wait(lk, [&] { return S->reserved + extra <= S->budget || S->growing == S->running; });
-    S->reserved += extra;
-    q->bytes += extra;
+    auto extra = [&](int b) {
+        const size_t need = footprint(S, q->r, b);
+        return need > q->bytes ? need - q->bytes : (size_t)0;
+    };
+    S->cv_work.wait(lk, [&] { return S->reserved + extra(band) <= S->budget || S->growing == S->running; });
+    while (band > 1 && S->reserved + extra(band) > S->budget) band--;
+    const size_t x = extra(band);
+    S->reserved += x;
+    q->bytes += x;
     S->growing--;
     S->cv_work.notify_all();
+    return band;
 }
 
 static hicma_lik_result run(hicma_server *S, hicma_ctx *ctx, SvRequest *q)
@@ -153,11 +163,8 @@ static hicma_lik_result run(hicma_server *S, hicma_ctx *ctx, SvRequest *q)
         if (!rc) band = hicma_band_autotune_model(G.data(), NT, r.NB, SERVE_DENSE_TFLOPS, SERVE_LR_TFLOPS);
         if (band < 1) rc = band;
     }
-    if (!rc && band > 1) {
-        const size_t need = footprint(S, r, band);
-        if (need > q->bytes) grow(S, q, need - q->bytes);
-        rc = parsec_band_regenerate(ctx, HICMA_LOWER, A, NULL, NULL, NULL, band);
-    }
+    if (!rc && band > 1) band = grow(S, q, band);
+    if (!rc && band > 1) rc = parsec_band_regenerate(ctx, HICMA_LOWER, A, NULL, NULL, NULL, band);
     res.band = band;
     if (!rc) {
         int la = -1, tf = 0;

@@ -115,6 +115,8 @@ extern "C" unsigned long hicma_op_counts(char op, unsigned long a, unsigned long
 }
 
 // ---------------------------------------------------------------------------
+#define POOL_KEEP_BYTES ((uint64_t)0)  // freed bytes a stream-ordered pool keeps
+
 extern "C" int hicma_init_ex(hicma_ctx **ctx, int device, int flags, int cu_share)
 {
     if (!ctx) return HICMA_ERR_ARG;
@@ -139,13 +141,16 @@ extern "C" int hicma_init_ex(hicma_ctx **ctx, int device, int flags, int cu_shar
     }
     HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     if (c->pool) {
-        // keep freed blocks in the device's pool instead of returning them,
-        // and never let one context's allocation reuse another stream's
-        // freed block by making its stream wait for that one (the contexts
-        // of a server would then serialize)
+        // return freed blocks beyond POOL_KEEP_BYTES to the device at the
+        // next synchronization (the contexts of a server free and allocate
+        // blocks of every size; a pool that kept them outgrew the 288 GB of
+        // the device with 150 GB reserved by admission), and never let one
+        // context's allocation reuse another stream's freed block by making
+        // its stream wait for that one (the contexts of a server would then
+        // serialize)
         hipMemPool_t mp;
         HIPCHK(hipDeviceGetDefaultMemPool(&mp, device));
-        uint64_t thr = ~0ULL;
+        uint64_t thr = POOL_KEEP_BYTES;
         HIPCHK(hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &thr));
         int no = 0;
         HIPCHK(hipMemPoolSetAttribute(mp, hipMemPoolReuseAllowInternalDependencies, &no));
@@ -542,12 +547,20 @@ extern "C" int hicma_band_autotune_weighted(const int *G, int NT, int NB, double
 // Band choice by a time model: every factorization task of a band size b is
 // priced at the device rate of the kernel that would run it (dense tiles and
 // low-rank products on the matrix cores at dense_tflops; recompressions at
-// their reference op counts and lr_tflops, a hundredth of it for stacks
-// wider than the wave kernels' 128 columns), and the
-// b with the least total time wins.  A gemm task (m, n, k) changes kind only
-// where b passes m - n, n - k or m - k, so each task adds its time to at most
-// four ranges of b (a difference array): O(NT^3) in all.  The new rank of a
-// recompressed tile is estimated by its current rank.
+// their reference op counts and lr_tflops, MODEL_WIDE_SLOWDOWN times slower
+// for stacks wider than the wave kernels' 128 columns, which go to the
+// 256-thread kernel with an HBM-resident SVD), plus MODEL_STEP_2FLOW_S per
+// panel step for the dense-band kernels of the 2-flow form, and the b with
+// the least total time wins.  A gemm task (m, n, k) changes kind only where
+// b passes m - n, n - k or m - k, so each task adds its time to at most four
+// ranges of b (a difference array): O(NT^3) in all.  The rank a recompressed
+// tile reaches during the factorization is estimated as MODEL_RANK_GROWTH
+// times its generated rank.  The constants are fitted to measured
+// factorization times of 2D sqexp problems (tile 512, maxrank 256, N 16 k ..
+// 64 k, beta 0.02 .. 0.2) on MI355X.
+#define MODEL_WIDE_SLOWDOWN 300.0
+#define MODEL_STEP_2FLOW_S 5e-4
+#define MODEL_RANK_GROWTH 1.25
 extern "C" int hicma_band_autotune_model(const int *G, int NT, int NB, double dense_tflops, double lr_tflops)
 {
     if (!G || NT <= 0 || NB <= 0 || !(dense_tflops > 0) || !(lr_tflops > 0)) return HICMA_ERR_ARG;
@@ -579,16 +592,17 @@ extern "C" int hicma_band_autotune_model(const int *G, int NT, int NB, double de
         for (int n = k + 1; n < NT; n++)
             for (int m = n + 1; m < NT; m++) {
                 const int x = m - n, y = n - k, z = m - k;
-                const double ra = rk(m, k), rb = rk(n, k), rc = rk(m, n);
+                const double ra = rk(m, k), rb = rk(n, k), rc = MODEL_RANK_GROWTH * rk(m, n);
                 const double t_all_dense = 2 * nb * nb * nb * rd;
                 const double t_cd_bd = 4 * nb * nb * ra * rd;                      // C, B dense; A low rank
                 const double t_cd = (4 * nb * ra * rb + 2 * nb * nb * rb) * rd;    // C dense; A, B low rank
                 // all low rank, and C, A low rank with B dense: stacks wider
                 // than 128 columns (or B of rank > 64) leave the wave
-                // kernels for the 256-thread one, about 100 times slower
+                // kernels for the 256-thread one
                 const bool wide = (ra + rc > 128);
-                const double t_v = lr_case_v(ra, rb, rc) * ((wide || rb > 64) ? 100.0 * rl : rl);
-                const double t_iv = lr_case_v(ra, 0, rc) * (wide ? 100.0 * rl : rl) + 2 * nb * nb * ra * rd;
+                const double t_v = lr_case_v(ra, rb, rc) * ((wide || rb > 64) ? MODEL_WIDE_SLOWDOWN * rl : rl);
+                const double t_iv =
+                    lr_case_v(ra, 0, rc) * (wide ? MODEL_WIDE_SLOWDOWN * rl : rl) + 2 * nb * nb * ra * rd;
                 add(1, std::min(x, y), t_v);
                 if (x < y) {
                     add(x + 1, y, t_cd);
@@ -602,6 +616,7 @@ extern "C" int hicma_band_autotune_model(const int *G, int NT, int NB, double de
                 add(z + 1, NT, t_all_dense);
             }
     }
+    add(2, NT, MODEL_STEP_2FLOW_S * NT);
     int best = 1;
     double acc = 0.0, tbest = 0.0;
     for (int b = 1; b <= NT; b++) {
@@ -720,7 +735,7 @@ size_t hicma_desc_bytes(int N, int NB, int cap, int band, int ndim)
 {
     const size_t NT = (size_t)(N + NB - 1) / NB;
     return sizeof(double) * (size_t)ndim * N + sizeof(int) * NT * NT + sizeof(double) * NT * (NT - 1) / 2 * NB * 2 * cap +
-           sizeof(unsigned long long) * (NT * NT + 16) + sizeof(double) * (size_t)band * NT * NB * NB;
+           sizeof(unsigned long long) * (NT * NT + 16) + sizeof(double) * (size_t)band * NT * NB * NB + NT * NT;
 }
 
 // blocked Cholesky of diagonal tile k: per 64-column block, factor + invert

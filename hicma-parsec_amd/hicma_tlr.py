@@ -173,7 +173,7 @@ def band_autotune_model(G, NT, NB, dense_tflops=10.0, lr_tflops=3.5):
 
 
 # device rates of the serving loop's band model (TFLOP/s)
-SERVE_DENSE_TFLOPS = 10.0
+SERVE_DENSE_TFLOPS = 60.0
 SERVE_LR_TFLOPS = 3.5
 
 

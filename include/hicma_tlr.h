@@ -118,9 +118,10 @@ int hicma_band_autotune_weighted(const int *rank_array, int NT, int NB, double l
 
 /* Band choice by a time model of this device: the band size with the least
  * estimated factorization time, dense work priced at dense_tflops and the
- * recompressions (the reference's op counts) at lr_tflops (a hundredth of it
- * for stacks wider than 128 columns).  No counterpart in the reference; the
- * serving loop uses it. */
+ * recompressions (the reference's op counts, tile ranks grown by 1.25) at
+ * lr_tflops (300 times slower for stacks wider than 128 columns), plus 0.5 ms
+ * per panel step for a dense band.  No counterpart in the reference; the
+ * serving loop uses it with 60 and 3.5. */
 int hicma_band_autotune_model(const int *rank_array, int NT, int NB, double dense_tflops, double lr_tflops);
 /* parsec_band_regenerate (hicma_parsec/hicma_parsec.h:108-121): dense band
  * tiles regenerated in HBM; data / kernel / index as for STARSH_gen */
@@ -291,7 +292,11 @@ int hicma_tlr_loglik_noalloc(hicma_ctx *ctx, hicma_tlr_desc *A, double *z, doubl
  * stream and ncu/slots CUs each) run requests concurrently; a request is
  * admitted when a worker is free and its HBM footprint fits the budget
  * (FIFO; a later request may pass a head that does not fit, for at most 1 s).
- * A request's result does not depend on what else runs. */
+ * After generation a request widens its footprint to the dense band the time
+ * model picks (result `band`), waiting for memory; a request's result does
+ * not depend on what else runs, except when every running request is waiting
+ * to widen: one then takes the widest band that fits the budget (its result
+ * agrees within the factorization's accuracy, not bit for bit). */
 typedef struct hicma_server hicma_server;
 typedef struct {
     int N, NB, maxrank;           /* maxrank <= NB/2; 0 selects NB/2 */

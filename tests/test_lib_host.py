@@ -113,19 +113,19 @@ def _model_time_bruteforce(G, NT, NB, b, rd, rl):
     nb = float(NB)
     rd, rl = 1.0 / (rd * 1e12), 1.0 / (rl * 1e12)
 
-    def case_v(ra, rb, rc):
-        s = ra + rc
+    def case_v(ra, rb, rcf):
+        s, rc = int(ra + rcf), int(rcf)
         return (2 * H.op_counts('q', NB, s) + H.op_counts('m', ra, rb, NB) + H.op_counts('m', ra, NB, rb) +
                 H.op_counts('r', s, s, 2) + H.op_counts('s', s) + s * rc + H.op_counts('o', NB, rc, s, 1) +
                 H.op_counts('o', rc, NB, s, 2))
-    t = 0.0
+    t = 5e-4 * NT if b > 1 else 0.0   # per panel step of the 2-flow form
     for k in range(NT):
         for m in range(k + 1, NT):
             r = rk(m, k)
             t += (3 * nb * nb * r if m - k >= b else 2 * nb ** 3) * rd
         for n in range(k + 1, NT):
             for m in range(n + 1, NT):
-                ra, rb, rc = rk(m, k), rk(n, k), rk(m, n)
+                ra, rb, rc = rk(m, k), rk(n, k), 1.25 * rk(m, n)   # C's rank grown during the factorization
                 dA, dB, dC = m - k < b, n - k < b, m - n < b
                 wide = ra + rc > 128
                 if dA:
@@ -135,9 +135,9 @@ def _model_time_bruteforce(G, NT, NB, b, rd, rl):
                 elif dC:
                     t += (4 * nb * ra * rb + 2 * nb * nb * rb) * rd
                 elif dB:
-                    t += case_v(ra, 0, rc) * (100 * rl if wide else rl) + 2 * nb * nb * ra * rd
+                    t += case_v(ra, 0, rc) * (300 * rl if wide else rl) + 2 * nb * nb * ra * rd
                 else:
-                    t += case_v(ra, rb, rc) * (100 * rl if (wide or rb > 64) else rl)
+                    t += case_v(ra, rb, rc) * (300 * rl if (wide or rb > 64) else rl)
     return t
 
 
