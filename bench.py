@@ -42,8 +42,8 @@ HBM_PEAK_GBS = 8000.0
 # counters' reading of a device copy of known size (x2.0 and x1.0).
 C2_TRAFFIC_BYTES_PER_GROUP = 6.399e9
 # HBM the serving sweep's admission may reserve (of 288 GB): the rest covers
-# the stream-ordered pool's cached blocks and allocation granularity
-SERVE_BUDGET_GB = 150.0
+# what the stream-ordered pool holds beyond the reservations
+SERVE_BUDGET_GB = 200.0
 
 
 def log(*a):

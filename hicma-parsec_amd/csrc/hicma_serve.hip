@@ -22,6 +22,7 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -36,6 +37,8 @@ size_t hicma_solve_bytes(int N, int NB);  // hicma_solve.hip
 // about 3.5 TFLOP/s of reference op counts)
 #define SERVE_DENSE_TFLOPS 60.0
 #define SERVE_LR_TFLOPS 3.5
+#define SERVE_BAND_BYTES ((size_t)4 << 30)  // dense band memory a request may take
+#define SERVE_PENDING_RANK 160  // model rank of a tile the narrow sketch cannot vouch for (the second sketch's width)
 
 namespace {
 
@@ -73,14 +76,15 @@ struct hicma_server {
 
 // HBM a request needs with a dense band of `band` tile diagonals: the
 // descriptor, the context workspace (generation / factorization / solve share
-// one buffer), the vector
+// one buffer that grows from one to the next: the old block is freed in
+// stream order while the new one is allocated, so both are counted), the
+// vector
 static size_t footprint(hicma_server *S, const hicma_lik_request &r, int band)
 {
     const int cap = r.maxrank;
     size_t ws = hicma_work_bytes(S->probe, r.N, r.NB, cap, band);
     size_t sv = hicma_solve_bytes(r.N, r.NB);
-    return hicma_desc_bytes(r.N, r.NB, cap, band, 2) + (ws > sv ? ws : sv) + sizeof(double) * (size_t)r.N +
-           ((size_t)1 << 20);
+    return hicma_desc_bytes(r.N, r.NB, cap, band, 2) + ws + sv + sizeof(double) * (size_t)r.N + ((size_t)1 << 20);
 }
 
 static void publish(hicma_server *S, uint64_t id, const hicma_lik_result &res)
@@ -154,17 +158,27 @@ static hicma_lik_result run(hicma_server *S, hicma_ctx *ctx, SvRequest *q)
     double *dz = nullptr;
     int rc = 0, info = 0, band = 1;
     rc = hicma_desc_create(ctx, &A, r.N, r.NB, r.maxrank, HICMA_KIND_SQEXP_2D, r.sigma2, r.beta, r.nugget, r.seed);
-    if (!rc) rc = STARSH_gen(ctx, HICMA_LOWER, A, A, NULL, NULL, NULL, r.eps, r.maxrank, 1, 0, &info);
+    // generation in two stages: the band is chosen from the narrow sketch's
+    // ranks (a tile it cannot vouch for counted at SERVE_PENDING_RANK), so
+    // the wider sketches run only on the tiles that stay low rank
+    if (!rc) rc = hicma_gen_first(ctx, A, r.eps, r.maxrank, &info);
     if (!rc && info) rc = info;
     if (!rc) {
         const int NT = (r.N + r.NB - 1) / r.NB;
         std::vector<int> G((size_t)NT * NT);
         rc = parsec_rank_gather(ctx, A, G.data(), 1);
+        for (int &g : G)
+            if (g == HICMA_RANK_PENDING) g = SERVE_PENDING_RANK;
         if (!rc) band = hicma_band_autotune_model(G.data(), NT, r.NB, SERVE_DENSE_TFLOPS, SERVE_LR_TFLOPS);
         if (band < 1) rc = band;
+        // at most SERVE_BAND_BYTES of dense band per request
+        const int bmax = (int)std::max<size_t>(1, SERVE_BAND_BYTES / ((size_t)NT * r.NB * r.NB * sizeof(double)));
+        if (band > bmax) band = bmax;
     }
     if (!rc && band > 1) band = grow(S, q, band);
     if (!rc && band > 1) rc = parsec_band_regenerate(ctx, HICMA_LOWER, A, NULL, NULL, NULL, band);
+    if (!rc) rc = hicma_gen_finish(ctx, A, r.eps, r.maxrank, &info);
+    if (!rc && info) rc = info;
     res.band = band;
     if (!rc) {
         int la = -1, tf = 0;
@@ -197,6 +211,9 @@ static void worker(hicma_server *S, hicma_ctx *ctx)
             if (!q) break;
         }
         const hicma_lik_result res = run(S, ctx, q);
+        // the request's memory is freed in stream order: wait for the frees
+        // before its reservation goes to another request
+        (void)hipStreamSynchronize(ctx->stream);
         {
             std::lock_guard<std::mutex> lk(S->mu);
             S->reserved -= q->bytes;

@@ -115,8 +115,6 @@ extern "C" unsigned long hicma_op_counts(char op, unsigned long a, unsigned long
 }
 
 // ---------------------------------------------------------------------------
-#define POOL_KEEP_BYTES ((uint64_t)0)  // freed bytes a stream-ordered pool keeps
-
 extern "C" int hicma_init_ex(hicma_ctx **ctx, int device, int flags, int cu_share)
 {
     if (!ctx) return HICMA_ERR_ARG;
@@ -141,16 +139,13 @@ extern "C" int hicma_init_ex(hicma_ctx **ctx, int device, int flags, int cu_shar
     }
     HIPCHK(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     if (c->pool) {
-        // return freed blocks beyond POOL_KEEP_BYTES to the device at the
-        // next synchronization (the contexts of a server free and allocate
-        // blocks of every size; a pool that kept them outgrew the 288 GB of
-        // the device with 150 GB reserved by admission), and never let one
-        // context's allocation reuse another stream's freed block by making
-        // its stream wait for that one (the contexts of a server would then
-        // serialize)
+        // keep freed blocks in the device's pool instead of returning them,
+        // and never let one context's allocation reuse another stream's
+        // freed block by making its stream wait for that one (the contexts
+        // of a server would then serialize)
         hipMemPool_t mp;
         HIPCHK(hipDeviceGetDefaultMemPool(&mp, device));
-        uint64_t thr = POOL_KEEP_BYTES;
+        uint64_t thr = ~0ULL;
         HIPCHK(hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &thr));
         int no = 0;
         HIPCHK(hipMemPoolSetAttribute(mp, hipMemPoolReuseAllowInternalDependencies, &no));
@@ -384,6 +379,51 @@ static int apply_problem(hicma_tlr_desc *A, const void *data, const hicma_kernel
     return 0;
 }
 
+// The compression passes first .. last of STARSH_gen over the off-band
+// tiles (outside A's current band): pass 0 compresses every tile with the
+// narrow sketch, passes 1 and 2 again, at wider sketches up to the full
+// width maxrank + GEN_OVS, exactly the tiles the narrower one could not vouch
+// for (rank map GEN_REDO).
+static int gen_lr(hicma_tlr_desc *A, double tol, int maxrank, int first, int last)
+{
+    hicma_ctx *ctx = A->ctx;
+    TLR &t = A->t;
+    int rc = 0;
+    int NB = t.NB, p = maxrank + 10;
+    if (p > NB) p = NB;
+    long long ntask = (long long)t.NT * (t.NT - 1) / 2;
+    if (ntask > 0 && t.NT > t.band_size) {
+        std::vector<double> om((size_t)NB * p);
+        sketch(NB, p, SKETCH_SEED, om.data());
+        double *d_om = nullptr;
+        HIPCHK(ctx_malloc(ctx, (void **)&d_om, sizeof(double) * om.size()));
+        HIPCHK(hipMemcpy(d_om, om.data(), sizeof(double) * om.size(), hipMemcpyHostToDevice));  // stream idle (ctx_malloc)
+        GenWs g;
+        g.p = p;
+        g.stride = gen_slot_doubles(NB, p, t.cap);
+        int ns = nslots_for(ctx, g.stride * sizeof(double), ntask);
+        if ((rc = hicma_ensure_ws(ctx, g.stride * sizeof(double) * ns))) return rc;
+        size_t lds = sizeof(double) * (BLK_GEMM_SMEM + BLK_RED_SMEM + 2 * QRB * QRB + 3 * (size_t)p + 16);
+        HIPCHK(hipFuncSetAttribute((const void *)k_compress, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        // narrow sketch first; tiles whose rank it cannot vouch for again at
+        // a wider one, up to the full width
+        const int ladder[3] = {GEN_FIRST_SKETCH, GEN_SECOND_SKETCH, p};
+        int prev = 0;
+        for (int pass = 0; pass <= last; pass++) {
+            const int pw = ladder[pass] < p ? ladder[pass] : p;
+            if (pw <= prev) continue;
+            if (pass >= first)
+                hipLaunchKernelGGL(k_compress, dim3(ns), dim3(BS), lds, ctx->stream, t, (const double *)d_om,
+                                   ctx->ws, g, pw, pass > 0 ? 1 : 0, tol, maxrank, ctx->d_info);
+            prev = pw;
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx_free(ctx, d_om));
+    }
+    return 0;
+}
+
 extern "C" int STARSH_gen(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr_desc *Ar, void *data,
                           hicma_kernel *kernel, hicma_index *index, double tol, int maxrank, int band_size,
                           int send_full_tile, int *info)
@@ -403,37 +443,47 @@ extern "C" int STARSH_gen(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, hicma_tlr
     A->demoted = 0;
     HIPCHK(hipMemsetAsync(ctx->d_info, 0, 4 * sizeof(int), ctx->stream));
     if ((rc = gen_band_tiles(A))) return rc;
-    int NB = t.NB, p = maxrank + 10;
-    if (p > NB) p = NB;
-    long long ntask = (long long)t.NT * (t.NT - 1) / 2;
-    if (ntask > 0 && t.NT > band_size) {
-        std::vector<double> om((size_t)NB * p);
-        sketch(NB, p, SKETCH_SEED, om.data());
-        double *d_om = nullptr;
-        HIPCHK(ctx_malloc(ctx, (void **)&d_om, sizeof(double) * om.size()));
-        HIPCHK(hipMemcpy(d_om, om.data(), sizeof(double) * om.size(), hipMemcpyHostToDevice));  // stream idle (ctx_malloc)
-        GenWs g;
-        g.p = p;
-        g.stride = gen_slot_doubles(NB, p, t.cap);
-        int ns = nslots_for(ctx, g.stride * sizeof(double), ntask);
-        if ((rc = hicma_ensure_ws(ctx, g.stride * sizeof(double) * ns))) return rc;
-        size_t lds = sizeof(double) * (BLK_GEMM_SMEM + BLK_RED_SMEM + 2 * QRB * QRB + 3 * (size_t)p + 16);
-        HIPCHK(hipFuncSetAttribute((const void *)k_compress, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        // narrow sketch first; tiles whose rank it cannot vouch for again at
-        // a wider one, up to the full width
-        const int ladder[3] = {GEN_FIRST_SKETCH, GEN_SECOND_SKETCH, p};
-        int prev = 0;
-        for (int pass = 0; pass < 3; pass++) {
-            const int pw = ladder[pass] < p ? ladder[pass] : p;
-            if (pw <= prev) continue;
-            hipLaunchKernelGGL(k_compress, dim3(ns), dim3(BS), lds, ctx->stream, t, (const double *)d_om, ctx->ws,
-                               g, pw, prev > 0 ? 1 : 0, tol, maxrank, ctx->d_info);
-            prev = pw;
-        }
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        HIPCHK(ctx_free(ctx, d_om));
-    }
+    if ((rc = gen_lr(A, tol, maxrank, 0, 2))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int hinfo[4];
+    HIPCHK(hipMemcpy(hinfo, ctx->d_info, 4 * sizeof(int), hipMemcpyDeviceToHost));
+    if (info) *info = hinfo[0];
+    return 0;
+}
+
+// STARSH_gen in two stages for a caller that picks the band from the ranks
+// (the serving loop): the first compresses every off-band tile of band 1
+// with the narrow sketch and leaves the tiles it cannot vouch for at rank
+// HICMA_RANK_PENDING; after the caller's parsec_band_regenerate, the second
+// runs the wider passes on the pending tiles still outside the band.  The
+// off-band tiles end exactly as STARSH_gen leaves them; the pending tiles
+// the band took in are never compressed at the wider sketches.
+extern "C" int hicma_gen_first(hicma_ctx *ctx, hicma_tlr_desc *A, double tol, int maxrank, int *info)
+{
+    if (!ctx || !A || maxrank < 1 || maxrank > A->t.cap) return HICMA_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    TLR &t = A->t;
+    int rc = alloc_band(A, 1);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(t.rank, 0xFF, sizeof(int) * (size_t)t.NT * t.NT, ctx->stream));
+    HIPCHK(hipMemsetAsync(t.prec, LR_F64, (size_t)t.NT * t.NT, ctx->stream));
+    A->demoted = 0;
+    HIPCHK(hipMemsetAsync(ctx->d_info, 0, 4 * sizeof(int), ctx->stream));
+    if ((rc = gen_band_tiles(A))) return rc;
+    if ((rc = gen_lr(A, tol, maxrank, 0, 0))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    int hinfo[4];
+    HIPCHK(hipMemcpy(hinfo, ctx->d_info, 4 * sizeof(int), hipMemcpyDeviceToHost));
+    if (info) *info = hinfo[0];
+    return 0;
+}
+
+extern "C" int hicma_gen_finish(hicma_ctx *ctx, hicma_tlr_desc *A, double tol, int maxrank, int *info)
+{
+    if (!ctx || !A || maxrank < 1 || maxrank > A->t.cap) return HICMA_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = gen_lr(A, tol, maxrank, 1, 2);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     int hinfo[4];
     HIPCHK(hipMemcpy(hinfo, ctx->d_info, 4 * sizeof(int), hipMemcpyDeviceToHost));

@@ -37,7 +37,7 @@ EXPORTS = [
     "hicma_tlr_loglik", "hicma_dev_malloc", "hicma_dev_free", "hicma_memcpy", "hicma_tlr_loglik_noalloc",
     "hicma_init_ex", "hicma_server_create", "hicma_server_submit", "hicma_server_wait", "hicma_server_destroy",
     "hicma_normal", "hicma_factor_graph", "hicma_set_wave_min_tasks", "hicma_autotune_schedule",
-    "hicma_tlr_normf", "hicma_tlr_demote",
+    "hicma_tlr_normf", "hicma_tlr_demote", "hicma_gen_first", "hicma_gen_finish",
 ]
 
 _lib = None
@@ -132,6 +132,8 @@ def lib():
             "hicma_autotune_schedule": (i, [vp, vp, d, i, i, C.POINTER(i), C.POINTER(d)]),
             "hicma_tlr_normf": (i, [vp, C.POINTER(d), dp]),
             "hicma_tlr_demote": (i, [vp, d, d, ip]),
+            "hicma_gen_first": (i, [vp, vp, d, i, C.POINTER(i)]),
+            "hicma_gen_finish": (i, [vp, vp, d, i, C.POINTER(i)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -174,6 +176,9 @@ def band_autotune_model(G, NT, NB, dense_tflops=10.0, lr_tflops=3.5):
 
 # device rates of the serving loop's band model (TFLOP/s)
 SERVE_DENSE_TFLOPS = 60.0
+SERVE_PENDING_RANK = 160
+SERVE_BAND_BYTES = 4 << 30
+RANK_PENDING = -2
 SERVE_LR_TFLOPS = 3.5
 
 
@@ -415,6 +420,21 @@ class TLRMatrix:
                                 idx.ctypes.data if idx is not None else None, tol, maxrank or self.maxrank,
                                 band_size, 0, C.byref(info)), "STARSH_gen")
         self.band = band_size
+        return info.value
+
+    def generate_first(self, tol=1e-8, maxrank=None):
+        """hicma_gen_first: band 1, narrow sketch only (pending tiles at RANK_PENDING)."""
+        info = C.c_int(0)
+        _check(lib().hicma_gen_first(self.ctx.h, self.h, tol, maxrank or self.maxrank, C.byref(info)),
+               "hicma_gen_first")
+        self.band = 1
+        return info.value
+
+    def generate_finish(self, tol=1e-8, maxrank=None):
+        """hicma_gen_finish: the wider sketches for the pending tiles outside the band."""
+        info = C.c_int(0)
+        _check(lib().hicma_gen_finish(self.ctx.h, self.h, tol, maxrank or self.maxrank, C.byref(info)),
+               "hicma_gen_finish")
         return info.value
 
     def rank_gather(self, band=None):

@@ -123,6 +123,17 @@ int hicma_band_autotune_weighted(const int *rank_array, int NT, int NB, double l
  * per panel step for a dense band.  No counterpart in the reference; the
  * serving loop uses it with 60 and 3.5. */
 int hicma_band_autotune_model(const int *rank_array, int NT, int NB, double dense_tflops, double lr_tflops);
+/* STARSH_gen in two stages, for callers that choose the band from the ranks
+ * (the serving loop).  No counterpart in the reference.  hicma_gen_first:
+ * STARSH_gen with band 1 and only the narrow first sketch; tiles whose rank it
+ * cannot vouch for hold HICMA_RANK_PENDING in the rank map.  Then the caller
+ * may call parsec_band_regenerate; hicma_gen_finish compresses the pending
+ * tiles still outside the band at the wider sketches.  Off-band tiles end as
+ * STARSH_gen leaves them; pending tiles the band took in are never
+ * compressed at the wider sketches.  info as for STARSH_gen. */
+#define HICMA_RANK_PENDING (-2)
+int hicma_gen_first(hicma_ctx *ctx, hicma_tlr_desc *A, double tol, int maxrank, int *info);
+int hicma_gen_finish(hicma_ctx *ctx, hicma_tlr_desc *A, double tol, int maxrank, int *info);
 /* parsec_band_regenerate (hicma_parsec/hicma_parsec.h:108-121): dense band
  * tiles regenerated in HBM; data / kernel / index as for STARSH_gen */
 int parsec_band_regenerate(hicma_ctx *ctx, int uplo, hicma_tlr_desc *A, void *data, hicma_kernel *kernel,

@@ -38,11 +38,15 @@ def test_alone_equals_mix_equals_direct():
     with H.Context(0) as ctx:
         for a, (N, s2, b, sd) in zip(alone[:3], REQS[:3]):
             g = H.TLRMatrix(ctx, N, 512, 256, H.SQEXP_2D, s2, b, 1e-2, sd)
-            assert g.generate(1e-8, 256, 1) == 0
-            band = H.band_autotune_model(g.rank_gather(), g.NT, g.NB, H.SERVE_DENSE_TFLOPS,
-                                         H.SERVE_LR_TFLOPS)  # the server's band choice
+            assert g.generate_first(1e-8, 256) == 0
+            G = g.rank_gather()
+            G[G == H.RANK_PENDING] = H.SERVE_PENDING_RANK
+            band = H.band_autotune_model(G, g.NT, g.NB, H.SERVE_DENSE_TFLOPS,
+                                         H.SERVE_LR_TFLOPS)  # the server's band choice,
+            band = min(band, max(1, H.SERVE_BAND_BYTES // (g.NT * g.NB * g.NB * 8)))  # within its band memory
             if band != 1:
                 g.regenerate(band)
+            assert g.generate_finish(1e-8, 256) == 0
             assert g.potrf(1e-8, 0, 256) == 0
             ld, quad, ll = g.loglik(H.normal(100 + sd, N))
             assert (ld, quad) == (a["logdet"], a["quad"]) and band == a["band"]
@@ -97,3 +101,31 @@ def test_many_waiters():
         for x in th:
             x.join()
     assert all(out[t]["status"] == 0 for t in ts)
+
+
+@pytest.mark.parametrize("N,beta", [(12288, 0.03), (8192, 0.1)])
+def test_staged_generation_equals_starsh_gen(ctx, N, beta):
+    """hicma_gen_first + hicma_gen_finish leave every off-band tile as
+    STARSH_gen does (the same passes on the same tiles): bit for bit at band
+    1, and outside the band after a parsec_band_regenerate in between."""
+    ref = H.TLRMatrix(ctx, N, 512, 256, H.SQEXP_2D, 1.0, beta, 1e-2, 3)
+    assert ref.generate(1e-8, 256, 1) == 0
+    R = ref.rank_map()
+    for band in (1, 4):
+        g = H.TLRMatrix(ctx, N, 512, 256, H.SQEXP_2D, 1.0, beta, 1e-2, 3)
+        assert g.generate_first(1e-8, 256) == 0
+        pending = int(np.sum(g.rank_map() == H.RANK_PENDING))
+        if band > 1:
+            g.regenerate(band)
+        assert g.generate_finish(1e-8, 256) == 0
+        Gm = g.rank_map()
+        for m in range(g.NT):
+            for n in range(m - band + 1):
+                assert Gm[m, n] == R[m, n], (band, m, n)
+                if R[m, n] > 0:
+                    (U1, V1), _ = g.get_tile(m, n)
+                    (U0, V0), _ = ref.get_tile(m, n)
+                    assert np.array_equal(U1, U0) and np.array_equal(V1, V0), (band, m, n)
+        print(f"N={N} beta={beta} band {band}: {pending} tiles pending after the first sketch")
+        g.close()
+    ref.close()
