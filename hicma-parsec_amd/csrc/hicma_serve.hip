@@ -47,7 +47,8 @@ struct SvRequest {
     hicma_lik_request r;
     std::vector<double> z;
     double t_submit, t_start;
-    size_t bytes;  // reserved HBM
+    size_t bytes;     // reserved HBM
+    bool exclusive;   // retried after running out of HBM: runs with nothing else
 };
 
 double now_s()
@@ -70,6 +71,7 @@ struct hicma_server {
     uint64_t next_id;
     int growing;   // requests waiting for band memory
     int running;   // admitted, not finished
+    bool exclusive_running;  // an exclusive request is running: admit nothing
     bool stop;
     std::vector<std::thread> workers;
 };
@@ -96,10 +98,19 @@ static void publish(hicma_server *S, uint64_t id, const hicma_lik_result &res)
 // with S->mu held: the next request to run, or null
 static SvRequest *admit(hicma_server *S)
 {
-    if (S->growing > 0) return nullptr;
+    if (S->growing > 0 || S->exclusive_running) return nullptr;
     const double t = now_s();
     for (auto it = S->queue.begin(); it != S->queue.end();) {
         SvRequest *q = *it;
+        if (q->exclusive) {  // waits at the head until nothing runs
+            if (S->running > 0) return nullptr;
+            S->queue.erase(it);
+            S->reserved += q->bytes;
+            S->running++;
+            S->exclusive_running = true;
+            q->t_start = t;
+            return q;
+        }
         if (q->bytes > S->budget) {  // can never fit
             hicma_lik_result res{};
             res.status = HICMA_ERR_NOMEM;
@@ -210,6 +221,11 @@ static void worker(hicma_server *S, hicma_ctx *ctx)
             S->cv_work.wait(lk, [&] { return (q = admit(S)) != nullptr || (S->stop && S->queue.empty()); });
             if (!q) break;
         }
+        if (q->exclusive) {
+            // nothing else runs: return the pool's cached blocks to the device
+            hipMemPool_t mp;
+            if (hipDeviceGetDefaultMemPool(&mp, ctx->device) == hipSuccess) (void)hipMemPoolTrimTo(mp, 0);
+        }
         const hicma_lik_result res = run(S, ctx, q);
         // the request's memory is freed in stream order: wait for the frees
         // before its reservation goes to another request
@@ -218,7 +234,19 @@ static void worker(hicma_server *S, hicma_ctx *ctx)
             std::lock_guard<std::mutex> lk(S->mu);
             S->reserved -= q->bytes;
             S->running--;
-            publish(S, q->id, res);
+            if (q->exclusive) S->exclusive_running = false;
+            if (res.status == HICMA_ERR_NOMEM && !q->exclusive) {
+                // the device ran out of HBM that the reservations did not
+                // account for (blocks the stream-ordered pool keeps): run it
+                // again, alone, first in line
+                (void)hipGetLastError();
+                q->exclusive = true;
+                q->bytes = footprint(S, q->r, 1);
+                S->queue.push_front(q);
+                q = nullptr;
+            } else {
+                publish(S, q->id, res);
+            }
             S->cv_work.notify_all();
         }
         delete q;

@@ -307,7 +307,8 @@ int hicma_tlr_loglik_noalloc(hicma_ctx *ctx, hicma_tlr_desc *A, double *z, doubl
  * model picks (result `band`), waiting for memory; a request's result does
  * not depend on what else runs, except when every running request is waiting
  * to widen: one then takes the widest band that fits the budget (its result
- * agrees within the factorization's accuracy, not bit for bit). */
+ * agrees within the factorization's accuracy, not bit for bit).  A request
+ * whose allocation fails (HICMA_ERR_NOMEM) runs once more, alone. */
 typedef struct hicma_server hicma_server;
 typedef struct {
     int N, NB, maxrank;           /* maxrank <= NB/2; 0 selects NB/2 */
