@@ -104,3 +104,25 @@ def test_harness_check_run():
     # critical-path times (potrf, trsm, syrk) are measured, not zero
     tp = re.search(r"Total_critical_path_time (\S+), potrf (\S+), trsm (\S+), syrk (\S+)", out)
     assert tp and all(float(x) > 0 for x in tp.groups())
+
+
+def test_harness_mixed_check_run():
+    """testing_dpotrf_tlr --mixed -x (N 2,700, tile 270, band auto-tuned as in
+    the run above): the factor's off-band tiles below eps ||A||_F are held in
+    FP32/BF16 after the factorization (the far corner tile at this size) and
+    the reference's checks pass on that factor (exit status 0)."""
+    exe = os.path.join(ROOT, "hicma-parsec_amd", "testing_dpotrf_tlr")
+    r = subprocess.run([exe, "-N", "2700", "-t", "270", "-e", "1e-8", "-u", "130", "-D", "2", "--nugget", "1e-2",
+                        "-x", "--mixed"], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-3000:])
+    print(r.stderr[-2000:])
+    assert r.returncode == 0
+    out = r.stdout
+    assert "Factorization is CORRECT" in out and "SAME matrices" in out
+    m = re.search(r"Mixed_precision\s+\|\|A\|\|_F= (\S+) thresholds (\S+) (\S+) tiles FP64= (\d+) FP32= (\d+) BF16= (\d+)",
+                  out)
+    assert m, "no Mixed_precision record"
+    n64, n32, n16 = (int(x) for x in m.groups()[3:])
+    assert n64 + n32 + n16 <= 10 * 9 // 2 and n32 + n16 >= 1
+    m = re.search(r"log det = (\S+) dense = (\S+) relative error = (\S+)", out)
+    assert m and float(m.group(3)) <= 1e-8
