@@ -78,6 +78,7 @@ def test_driver_usage():
     assert os.path.exists(exe)
     r = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "--fixedacc" in r.stderr
+    assert "--mixed" in r.stderr and "--check" in r.stderr and "--print-rank" in r.stderr
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
 
