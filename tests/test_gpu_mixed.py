@@ -16,16 +16,10 @@ import pytest
 
 import hicma_tlr as H
 import tlr_oracle as O
+from roundings import bf16_round, f32_round
 
 pytestmark = pytest.mark.gpu
 N, NB, U, EPS = 8100, 450, 150, 1e-8
-
-
-def bf16_round(x):
-    """double -> float32 (nearest even) -> upper 16 bits (nearest even), as a double."""
-    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
-    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint32) << np.uint32(16)
-    return u.view(np.float32).astype(np.float64)
 
 
 def make(ctx):
@@ -92,8 +86,7 @@ def test_demote(ctx, base, t32, t16):
             (U2, V2), r2 = h.get_tile(*key)
             assert r2 == r
             cls = want_cls[key]
-            rnd = (lambda x: x) if cls == 0 else ((lambda x: x.astype(np.float32).astype(np.float64)) if cls == 1
-                                                   else bf16_round)
+            rnd = (lambda x: x) if cls == 0 else (f32_round if cls == 1 else bf16_round)
             assert np.array_equal(U2, rnd(Ut)) and np.array_equal(V2, rnd(Vt)), (key, cls)
         # the solve reads the demoted tiles: same result as the oracle's FP64
         # solve on the widened values

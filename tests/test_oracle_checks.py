@@ -145,3 +145,26 @@ def test_tlr_trsm_equals_dense_solves(N, NB, auto):
     assert quad == pytest.approx(want_quad, rel=1e-6)
     sign, want_ld = np.linalg.slogdet(S)
     assert sign > 0 and o.logdet() == pytest.approx(want_ld, rel=1e-8)
+
+
+def test_storage_roundings_known_answers():
+    """The checker of the mixed-precision tests against values worked by
+    hand: BF16 keeps 8 significant bits (ulp 2^-7 at 1), ties go to the even
+    significand, and a double is rounded to FP32 first, so a value just above
+    a BF16 tie by less than half an FP32 ulp lands on the tie and goes to even."""
+    from roundings import bf16_round, f32_round
+    cases = [
+        (1.0, 1.0),
+        (1 + 2.0 ** -8, 1.0),                          # tie, even significand below
+        (1 + 3 * 2.0 ** -8, 1 + 2.0 ** -6),            # tie, even significand above
+        (1 + 2.0 ** -8 + 2.0 ** -20, 1 + 2.0 ** -7),   # above the tie
+        (1 + 2.0 ** -8 + 2.0 ** -30, 1.0),             # FP32 first: lands on the tie
+        (-2.5, -2.5),
+        (np.pi, 3.140625),
+        (2.0 ** -130, 2.0 ** -130),                    # FP32 subnormal, exact in BF16
+        (0.0, 0.0),
+    ]
+    x = np.array([c[0] for c in cases])
+    assert np.array_equal(bf16_round(x), np.array([c[1] for c in cases]))
+    y = np.array([1 + 2.0 ** -24, 1 + 3 * 2.0 ** -24, 1 + 2.0 ** -23, 0.1])
+    assert np.array_equal(f32_round(y), np.array([1.0, 1 + 2.0 ** -22, 1 + 2.0 ** -23, 0.10000000149011612]))
